@@ -1,0 +1,18 @@
+"""fivo2d -- MI355X-native 2+1D finite-volume hydro core (FiVoHydro.jl's src2d hot path).
+
+The compute runs in hand-written FP64 HIP kernels for gfx950 behind a C ABI
+(include/fivo2d.h, csrc/); this package is the host-side mirror of the
+reference's Julia API over that ABI.
+"""
+from ._abi import (FIVO_BC_OUTFLOW, FIVO_BC_PERIODIC, FIVO_SSPRK2, FIVO_SSPRK3, Lib, load_hip,
+                   HIP_LIB_PATH)
+from .model import (ConformalHQEOS, LatticeHRGEOS, Grid2D, make_grid2d, make_grid_2d,
+                    make_layout2d, build_model_2d, allocate_state, resolve_terms, T_MIN, E_FLOOR,
+                    HBARC)
+from .solver import Solver, FivoError
+from . import ic
+
+__all__ = ["FIVO_BC_OUTFLOW", "FIVO_BC_PERIODIC", "FIVO_SSPRK2", "FIVO_SSPRK3", "Lib", "load_hip",
+           "HIP_LIB_PATH", "ConformalHQEOS", "LatticeHRGEOS", "Grid2D", "make_grid2d",
+           "make_grid_2d", "make_layout2d", "build_model_2d", "allocate_state", "resolve_terms",
+           "Solver", "FivoError", "ic", "T_MIN", "E_FLOOR", "HBARC"]

@@ -1,0 +1,116 @@
+"""Initial conditions for the 2+1D solver, as primitive fields on the full grid.
+
+  gubser_*            ideal Gubser flow, src/gubser.jl:20-57 (analytic)
+  uniform             transversely uniform state, main2D.jl:173-182 (Bjorken)
+  hotspot_event       seeded lumpy event: Gaussian hot spots whose centres are
+                      drawn from a 2-D Woods-Saxon profile, Gamma(k=1) amplitudes,
+                      charm density from the binary-collision density of the same
+                      spots (the pattern of examples2d/04_fluctuating_event.jl:66-73)
+
+Every function returns arrays over all Ntot cells (ghosts included), flattened in
+the library's flat order i = ix*Nytot + iy, ready for Solver.init_from_prims.
+"""
+import math
+
+import numpy as np
+
+from .model import T_MIN
+
+
+def _posden(x):
+    return np.where(x > 1e-50, x, 1e-50)
+
+
+def gubser_den(tau, r, q):
+    q2 = q * q
+    return 1.0 + 2.0 * q2 * (tau * tau + r * r) + (q2 * q2) * ((tau * tau - r * r) ** 2)
+
+
+def gubser_vr(tau, r, q):
+    q2 = q * q
+    return (2.0 * q2 * tau * r) / _posden(1.0 + q2 * tau * tau + q2 * r * r)
+
+
+def gubser_ur(tau, r, q):
+    v = np.clip(gubser_vr(tau, r, q), -0.999999999, 0.999999999)
+    return v / np.sqrt(1.0 - v * v)
+
+
+def gubser_temperature(tau, r, q, Tscale):
+    qpos = max(abs(q), 1e-50)
+    return Tscale * (2.0 * qpos) ** (2 / 3) / (max(tau, 1e-50) ** (1 / 3) *
+                                                _posden(gubser_den(tau, r, q)) ** (1 / 3))
+
+
+def gubser_Tscale_from_center_T(tau0, q, Tc0):
+    qpos = max(abs(q), 1e-50)
+    fac = max(tau0, 1e-50) ** (1 / 3) * (1.0 + (qpos * tau0) ** 2) ** (2 / 3)
+    return max(Tc0, 1e-50) * fac / (2.0 * qpos) ** (2 / 3)
+
+
+def _mesh(g):
+    X, Y = np.meshgrid(g.xC, g.yC, indexing="ij")
+    return X, Y
+
+
+def gubser_ic(g, tau0=1.0, q=1.0, Tc0=0.6, alpha=-20.0):
+    """Analytic ideal Gubser state at tau0 (test/test_gubser2d.jl:97-108)."""
+    Tscale = gubser_Tscale_from_center_T(tau0, q, Tc0)
+    X, Y = _mesh(g)
+    r = np.hypot(X, Y)
+    T = gubser_temperature(tau0, r, q, Tscale)
+    ur = gubser_ur(tau0, r, q)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        ux = np.where(r > 0, ur * X / np.where(r > 0, r, 1.0), 0.0)
+        uy = np.where(r > 0, ur * Y / np.where(r > 0, r, 1.0), 0.0)
+    A = np.full_like(T, alpha)
+    return dict(T=T.ravel(), alpha=A.ravel(), ux=ux.ravel(), uy=uy.ravel()), Tscale
+
+
+def uniform(g, T0=0.5, alpha0=-4.2):
+    n = g.Ntot
+    return dict(T=np.full(n, T0), alpha=np.full(n, alpha0), ux=np.zeros(n), uy=np.zeros(n))
+
+
+# ---- seeded hot-spot event generator --------------------------------------------
+def _woods_saxon_centres(rng, n, R, a):
+    """Rejection-sample n points from rho(r) ~ 1/(1+exp((r-R)/a)) on the plane."""
+    out = np.empty((0, 2))
+    rmax = R + 10 * a
+    while out.shape[0] < n:
+        m = 4 * (n - out.shape[0]) + 16
+        xy = rng.uniform(-rmax, rmax, size=(m, 2))
+        r = np.hypot(xy[:, 0], xy[:, 1])
+        acc = rng.uniform(0, 1, size=m) < 1.0 / (1.0 + np.exp((r - R) / a))
+        out = np.vstack([out, xy[acc & (r <= rmax)]])
+    return out[:n]
+
+
+def hotspot_event(g, seed, nspots=150, sigma=0.5, Tpeak=0.45, R=6.6, a=0.55, eos=None,
+                  alpha_floor=-30.0):
+    """Seeded lumpy event on grid `g`.
+
+    Entropy-like density s(x) = sum_k A_k G_sigma(x - x_k) with centres from a
+    2-D Woods-Saxon (R, a) and A_k ~ Gamma(k=1). T ~ s^(1/3), scaled so max T =
+    Tpeak, floored at T_MIN. The charm fugacity alpha = log(rho_bin / rho_ref)
+    follows the binary-collision density rho_bin = sum_{k<l} A_k A_l G(x_k) G(x_l)
+    in the product approximation rho_bin ~ s^2, normalised so alpha = 0 at the
+    hottest cell and floored at alpha_floor. Dissipative fields start at zero.
+    """
+    rng = np.random.Generator(np.random.Philox(seed))
+    cen = _woods_saxon_centres(rng, nspots, R, a)
+    amp = rng.gamma(1.0, 1.0, size=nspots)
+    X, Y = _mesh(g)
+    s = np.zeros_like(X)
+    inv2s2 = 1.0 / (2 * sigma * sigma)
+    for (cx, cy), A in zip(cen, amp):
+        d2 = (X - cx) ** 2 + (Y - cy) ** 2
+        s += A * np.exp(-d2 * inv2s2)
+    s = s / s.max()
+    T = Tpeak * np.cbrt(s)
+    T = np.maximum(T, T_MIN)
+    with np.errstate(divide="ignore"):
+        alpha = np.where(s > 0, 2.0 * np.log(np.maximum(s, 1e-300)), alpha_floor)
+    alpha = np.maximum(alpha, alpha_floor)
+    n = g.Ntot
+    return dict(T=T.ravel(), alpha=alpha.ravel(), ux=np.zeros(n), uy=np.zeros(n))

@@ -1,0 +1,224 @@
+/*
+ * fivo2d.h -- C ABI of the MI355X (gfx950) 2+1D finite-volume hydro core.
+ *
+ * The library replaces the bodies of FiVoHydro.jl's 2+1D sweep and stepper
+ * functions -- the functions `run_sim_2d!` (main2D.jl:371-561) calls with the
+ * `(U, g, tau, model, work)` convention -- with hand-written FP64 HIP kernels.
+ * Every entry point below names the reference function it replaces.
+ *
+ * Conventions
+ *  - Host state arrays use the reference's layout: U[a + nv*i] (Julia
+ *    column-major Matrix(nv, Ntot)), flat cell index i = ix*Nytot + iy,
+ *    0-based, y fastest (src2d/grid2d.jl:9-16,85). The device keeps a
+ *    structure-of-arrays copy; the layout swap happens inside the copy calls.
+ *  - The caller owns every host pointer. The library copies in/out inside the
+ *    call and keeps no host pointer. The context owns all device memory.
+ *  - Status codes: 0 = ok; >0 = numerical outcome (e.g. FIVO_STEP_FAILED when a
+ *    step still fails after dt halving, mirroring `ok=false` in main2D.jl:466-473);
+ *    <0 = API or HIP error, text via fivo_last_error().
+ *  - A context is single-threaded and owns one HIP stream. Calls return once
+ *    their results are visible to the host.
+ */
+#ifndef FIVO2D_H
+#define FIVO2D_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FIVO_ABI_VERSION 1
+
+/* ---- status codes ---- */
+#define FIVO_OK              0
+#define FIVO_STEP_FAILED     1   /* step failed after max_halve halvings */
+#define FIVO_ERR_ARG        -1   /* invalid argument / refused knob */
+#define FIVO_ERR_HIP        -2   /* HIP runtime error */
+#define FIVO_ERR_STATE      -3   /* call sequence error (e.g. no state set) */
+
+/* ---- equation of state kinds (src/eos.jl) ---- */
+#define FIVO_EOS_CONFORMAL_HQ 0  /* ConformalHQEOS, src/eos.jl:39-84 */
+#define FIVO_EOS_LATTICE_HRG  1  /* LatticeHRGEOS,  src/eos.jl:318-519 */
+
+/* ---- boundary conditions (src2d/bc2d.jl:14) ---- */
+#define FIVO_BC_OUTFLOW  0
+#define FIVO_BC_PERIODIC 1
+
+/* ---- integrators (src2d/timestepper2d.jl:74,148) ---- */
+#define FIVO_SSPRK2 2
+#define FIVO_SSPRK3 3
+
+/* ---- Terms switches (src/terms.jl:82-106), bit k = field k of `Terms` ---- */
+#define FIVO_T_NU_GRADALPHA    (1u << 0)
+#define FIVO_T_FM_GRADT        (1u << 1)
+#define FIVO_T_FM_INERTIAL     (1u << 2)
+#define FIVO_T_FM_NU_GRADU     (1u << 3)
+#define FIVO_T_FM_EXPANSION    (1u << 4)
+#define FIVO_T_FM_DLNH         (1u << 5)
+#define FIVO_T_M2_NU_GRADIENT  (1u << 6)
+#define FIVO_T_M2_BG_GRADU     (1u << 7)
+#define FIVO_T_M2_BG_DLNT      (1u << 8)
+#define FIVO_T_M2_BG_DALPHA    (1u << 9)
+#define FIVO_T_M2_EXPANSION    (1u << 10)
+#define FIVO_T_M2_PI_SIGMA     (1u << 11)
+#define FIVO_T_M2_PIQ_SIGMA    (1u << 12)
+#define FIVO_T_M2_VORTICITY    (1u << 13)
+#define FIVO_T_M2_PROJECTOR    (1u << 14)
+#define FIVO_T_M2_ACCEL_NU     (1u << 15)
+#define FIVO_T_M2_NU_GRADTH    (1u << 16)
+#define FIVO_T_SHEAR_VORTICITY (1u << 17)
+/* `Terms()` defaults: everything on except the two vorticity couplings */
+#define FIVO_TERMS_DEFAULT (0x3FFFFu & ~(FIVO_T_M2_VORTICITY | FIVO_T_SHEAR_VORTICITY))
+
+/* Grid2D (src2d/grid2d.jl:19-61): box [-xmax,xmax] x [-ymax,ymax], nghost = 3. */
+typedef struct fivo_grid_desc {
+    int32_t nx, ny, nghost;
+    int32_t _pad;
+    double xmax, ymax;
+} fivo_grid_desc;
+
+/* EOS parameters: ConformalHQEOS (src/eos.jl:39-46) and LatticeHRGEOS
+ * (src/eos.jl:318-347). Unused fields of the other kind are ignored. */
+typedef struct fivo_eos_desc {
+    int32_t kind;
+    int32_t hq_times_fmGeV3;
+    double g_eff, m_hq, g_hq;
+    double a1, a2, a3, a4, b1, b2, b3, b4, c, d;
+    double canon_factor;
+} fivo_eos_desc;
+
+/* IdealDiffVisc2DModel (src2d/primitives2d.jl:41-210) as a flat POD, with the
+ * transport models QGPViscosity / SimpleBulkViscosity (src/primitives.jl:47-108)
+ * folded in. Field meaning and defaults follow build_model_2d (main2D.jl:85-123). */
+typedef struct fivo_model_desc {
+    fivo_eos_desc eos;
+    int32_t with_charge, enable_shear, enable_bulk, enable_diff;
+    int32_t consistent_fm, consistent_m2;
+    int32_t advect_nu, relax_advect_nu, advect_Pi, relax_advect_Pi;
+    int32_t advect_pi, relax_advect_pi, relax_advect_m2;
+    int32_t vacuum_ramp_relax, shear_projected_deriv, diff_projected_deriv;
+    int32_t shear_constraint_project;   /* 1 = :project, 0 = :monitor */
+    uint32_t terms;                     /* FIVO_T_* bitmask */
+    /* shear */
+    double eta_over_s, tauShear_coeff, deltaShear_factor, taupi_pi_factor, pi_clip_factor;
+    /* bulk */
+    double zeta_over_s, tauPi_coeff, deltaPi_factor, lambda_Pi_pi_factor,
+           lambda_pi_Pi_factor, Pi_clip_factor;
+    /* charge diffusion */
+    double kappa_coeff, transport_mass, tauN_coeff, deltaN_factor, nu_clip_factor,
+           vacuum_n_lo, vacuum_n_hi;
+    /* domain and vacuum cut */
+    double r_domain, T_vac_cut, E_vac_cut;   /* E_vac_cut <= 0: derived from T_vac_cut */
+} fivo_model_desc;
+
+/* Host views of the Work2D arrays (src2d/work2d.jl:15-100). Each pointer is
+ * nullable and, when set, points at Ntot doubles (ok: Ntot bytes). */
+typedef struct fivo_work_view {
+    double *yT, *phi, *mu, *alpha, *ux, *uy, *P, *n, *e;
+    uint8_t *ok;
+    double *x0_yT, *x0_phi, *x0_ux, *x0_uy;
+    double *ux_prev, *uy_prev, *alpha_prev, *T_prev, *nux_prev, *nuy_prev;
+    double *shear_res;
+} fivo_work_view;
+
+/* Per-call counters of the primitive pass (rhs2d.jl:89-112). */
+typedef struct fivo_counts {
+    int64_t nprimfail, nvacuum;
+    double amax;
+} fivo_counts;
+
+/* run_sim_2d! options (main2D.jl:371-378) */
+typedef struct fivo_run_opts {
+    double tau0, taufinal, cfl, cfltau, dump_dt;
+    int32_t integrator, bc;
+    int32_t max_halve, max_mood;
+    int64_t max_steps;
+    int32_t reset_history;
+    int32_t _pad;
+    /* called at tau0, every dump_dt and at the end, like `on_dump(τ, U, work)`;
+     * the callback may call fivo_get_state / fivo_get_work on the context */
+    void (*on_dump)(void* user, double tau, void* ctx);
+    void* user;
+} fivo_run_opts;
+
+/* run_sim_2d! result NamedTuple (main2D.jl:558-560) */
+typedef struct fivo_run_result {
+    int32_t ok;
+    int32_t _pad;
+    double tau;
+    int64_t nsteps, nprimfail, nvacuum;
+    double max_shear_res, maxu, dQ, Q0, Q1, minPtot;
+    /* extra accounting, not in the reference NamedTuple */
+    int64_t nrhs, nhalvings, nmood;
+    int64_t newton_iters;
+} fivo_run_result;
+
+typedef struct fivo_ctx fivo_ctx;
+
+/* ---- lifetime ---------------------------------------------------------- */
+/* build_model_2d (main2D.jl:85-123) + make_grid2d + allocate_state + make_work.
+ * Refuses the knobs reject_unwired_knobs_2d refuses (src2d/primitives2d.jl:246-272). */
+int fivo_create(const fivo_grid_desc* g, const fivo_model_desc* m, fivo_ctx** out);
+void fivo_destroy(fivo_ctx* ctx);
+const char* fivo_last_error(const fivo_ctx* ctx);   /* ctx may be NULL */
+int fivo_nvars(const fivo_ctx* ctx);
+int64_t fivo_ntot(const fivo_ctx* ctx);
+double fivo_e_vac_cut(const fivo_ctx* ctx);         /* derived E_vac_cut (main2D.jl:100-105) */
+
+/* ---- state and work ---------------------------------------------------- */
+int fivo_set_state(fivo_ctx* ctx, const double* U, int nv, int64_t ntot);
+int fivo_get_state(fivo_ctx* ctx, double* U, int nv, int64_t ntot);
+int fivo_set_work(fivo_ctx* ctx, const fivo_work_view* w);
+int fivo_get_work(fivo_ctx* ctx, fivo_work_view* w);
+/* NaN-fill the warm starts and histories (main2D.jl:407-415) */
+int fivo_reset_history(fivo_ctx* ctx);
+
+/* ---- initial conditions ------------------------------------------------ */
+/* set_cell! on every cell (main2D.jl:142-154) from host arrays of Ntot values
+ * (ux/uy/Pi/pixx/pixy/piyy/nux/nuy nullable = 0), then finalize_ic!
+ * (main2D.jl:164-171). nbad = cells whose prim_to_cons failed. */
+int fivo_init_from_prims(fivo_ctx* ctx, double tau0, const double* T, const double* alpha,
+                         const double* ux, const double* uy, const double* Pi,
+                         const double* pixx, const double* pixy, const double* piyy,
+                         const double* nux, const double* nuy, int bc, int64_t* nbad);
+/* finalize_ic! alone (main2D.jl:164-171) on the current state */
+int fivo_finalize_ic(fivo_ctx* ctx, double tau0, int bc);
+
+/* ---- sweeps -------------------------------------------------------------- */
+/* apply_bc_2d! (src2d/bc2d.jl:14-65) */
+int fivo_apply_bc(fivo_ctx* ctx, int bc);
+/* rhs_2d! (src2d/rhs2d.jl:169-249). force_first_order: nullable Ntot bytes.
+ * dU: nullable host Ntot*nv doubles (reference layout). */
+int fivo_rhs(fivo_ctx* ctx, double tau, int bc, const uint8_t* force_first_order,
+             double* dU, fivo_counts* counts);
+/* update_primitives_2d! (src2d/rhs2d.jl:57-65) */
+int fivo_update_primitives(fivo_ctx* ctx, double tau, int bc, fivo_counts* counts);
+/* sanitize_stage_2d! (src2d/floors2d.jl:218-227) */
+int fivo_sanitize(fivo_ctx* ctx, double tau, int bc);
+/* state_ok_2d (src2d/timestepper2d.jl:56-66); nbad = inadmissible interior cells */
+int fivo_state_ok(fivo_ctx* ctx, double tau, int* ok, int64_t* nbad);
+/* compute_dt_2d (src2d/timestepper2d.jl:37-47) from the last RHS's amax */
+int fivo_compute_dt(fivo_ctx* ctx, double tau, double cfl, double cfltau, double* dt);
+/* step_ssprk2_2d! / step_ssprk3_2d! (src2d/timestepper2d.jl:74-188) */
+int fivo_step(fivo_ctx* ctx, int integrator, double tau, double dtau, int bc,
+              int max_halve, int max_mood, int* ok, double* dtau_used);
+/* relax_dissipative_2d! (src2d/dissipation2d.jl:293-363) */
+int fivo_relax(fivo_ctx* ctx, double tau, double dtau);
+/* project_shear! (main2D.jl:332-358) */
+int fivo_project_shear(fivo_ctx* ctx, double* max_rel, double* mean_rel);
+/* run_sim_2d! (main2D.jl:371-561), device resident */
+int fivo_run(fivo_ctx* ctx, const fivo_run_opts* o, fivo_run_result* r);
+
+/* ---- timing hooks for benchmarks --------------------------------------- */
+/* Average device time (ms) of the last fivo_run's kernels by category, measured
+ * with HIP events on the context stream: 0 = primitive pass, 1 = faces+update,
+ * 2 = boundary fill, 3 = relaxation, 4 = everything. Returns count of launches. */
+int64_t fivo_kernel_time(const fivo_ctx* ctx, int category, double* total_ms);
+int fivo_enable_timing(fivo_ctx* ctx, int on);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FIVO2D_H */
