@@ -1,0 +1,127 @@
+// =============================================================================
+// fivo_ctx.h -- device context of the 2+1D hot path: HBM layout, launch
+// parameters and the kernel launchers shared by the C ABI (fivo_api.hip) and the
+// kernels (fivo_kernels.hip).
+//
+// HBM layout (structure of arrays): every cell field is a contiguous array of
+// `pitch` doubles (pitch = Ntot rounded up to 64), cell index c = ix*Nytot + iy
+// (y fastest, the reference's flat order). A state buffer is nv such arrays back
+// to back: field a of buffer U is U + a*pitch. A y-row of a field is contiguous,
+// so a wavefront of 64 lanes along y issues one 512-byte load per field.
+// =============================================================================
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+#include "fivo_device.h"
+#include "../../include/fivo2d.h"
+
+namespace fivo {
+
+// Model constants needed on the device (flat, passed by value as a kernel argument)
+struct Params {
+    int Nx, Ny, ng, Nxtot, Nytot, Ntot;
+    int pitch, nv;
+    int iNux, iNuy, iPi, iPixx, iPixy, iPiyy, iPieta;   // -1 when the sector is absent
+    int iPQ0;                                            // first charm second-moment row, -1 if absent
+    int hasNu, hasPi, hasShear, hasM2;
+    int has_rdom;
+    double dx, dy, xmin, ymin;
+    double Evac, rdom2;
+    fd::EosP eos;
+    // relaxation (src2d/dissipation2d.jl, src/primitives.jl:30-108)
+    int enable_shear, enable_bulk, enable_diff, consistent_fm, consistent_m2;
+    int relax_advect_pi, relax_advect_Pi, relax_advect_nu, relax_advect_m2;
+    int shear_projected_deriv, diff_projected_deriv, vacuum_ramp_relax;
+    uint32_t terms;
+    double eta_over_s, Cs, deltaShear_factor, taupi_pi_factor, pi_clip_factor;
+    double zeta_over_s, Czeta, deltaPi_factor, lambda_Pi_pi_factor, lambda_pi_Pi_factor, Pi_clip_factor;
+    double kappa_coeff, transport_mass, tauN_coeff, deltaN_factor, nu_clip_factor, vacuum_n_lo, vacuum_n_hi;
+};
+
+// Small device-resident scalars read back once per stage
+struct DevScalars {
+    unsigned long long amax_bits;   // max |signal speed| of the last RHS (positive double bits)
+    unsigned long long nbad;        // inadmissible cells of the last stage
+    unsigned long long nprimfail;   // last primitive pass
+    unsigned long long nvacuum;
+    unsigned long long newton_iters;// cumulative
+    unsigned long long shear_max_bits;
+    unsigned long long npending;    // near-bound cells awaiting the inversion test
+    unsigned long long pad;
+};
+
+// Per-cell work arrays on the device (Work2D, src2d/work2d.jl:15-100)
+struct DevWork {
+    double *yT, *phi, *mu, *alpha, *ux, *uy, *P, *n, *e, *vx, *vy;
+    uint8_t *ok, *bad, *force, *ftmp;
+    double *x0_yT, *x0_phi, *x0_ux, *x0_uy;
+    double *ux_prev, *uy_prev, *alpha_prev, *T_prev, *nux_prev, *nuy_prev;
+    double *shear_res;
+    double *gradAx, *gradAy;
+};
+
+enum { TK_PRIM = 0, TK_FACES = 1, TK_BC = 2, TK_RELAX = 3, TK_ALL = 4, TK_N = 5 };
+
+struct Ctx {
+    fivo_grid_desc gd;
+    fivo_model_desc md;
+    Params p;
+    int dev = 0;
+    hipStream_t stream = nullptr;
+    void* pool = nullptr;           // one device allocation carved into the arrays below
+    size_t pool_bytes = 0;
+    double* U[4] = {nullptr, nullptr, nullptr, nullptr};   // U[cur] = state, others stage outputs
+    int cur = 0;
+    double* dU = nullptr;           // RHS output for fivo_rhs
+    double* snap = nullptr;         // pre-relaxation snapshot (dissipation2d.jl:299-303)
+    double* partials = nullptr;     // reduction partials
+    int npartials = 0;
+    DevWork w;
+    DevScalars* sc = nullptr;       // device
+    DevScalars* sc_host = nullptr;  // pinned mirror
+    double amax_host = 0.0;         // last RHS amax, as read back
+    std::string err;
+    // tile shape of the fused face kernel (chosen by grid size or autotuned)
+    int tile = 0;
+    // timing
+    bool timing = false;
+    double t_ms[TK_N] = {0, 0, 0, 0, 0};
+    int64_t t_cnt[TK_N] = {0, 0, 0, 0, 0};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    std::vector<hipEvent_t> evpool;
+    std::vector<int> ev_cat;
+    int ev_used = 0;
+    // run accounting
+    int64_t nrhs = 0, nhalvings = 0, nmood = 0;
+    int64_t last_primfail = 0, last_vacuum = 0;
+};
+
+// ---- launchers (fivo_kernels.hip) ------------------------------------------
+struct FaceLaunch {
+    const double* Ub;      // base state (RK input = stage input)
+    const double* Uold;    // RK weight state (may equal Ub)
+    double* Uout;          // stage output (mode 1) -- never aliases Ub
+    double* dUout;         // RHS output (mode 0)
+    int mode;              // 0: write dU; 1: combine + sanitize + admissibility
+    int combine;           // 0: Uin + D k; 1: wold Uold + wnew (Uin + D k); 2: (Uold + 2 (Uin + D k))/3
+    double tau, tau_state, D, wold, wnew;
+    const uint8_t* force;  // MOOD first-order mask or null
+};
+hipError_t launch_bc(Ctx& c, double* U, int bc);
+hipError_t launch_prim(Ctx& c, const double* U, double tau);
+hipError_t launch_faces(Ctx& c, const FaceLaunch& f);
+hipError_t launch_expand(Ctx& c, const uint8_t* in, uint8_t* out);
+hipError_t launch_sanitize(Ctx& c, double* U, double tau, int use_P);
+hipError_t launch_admissible(Ctx& c, const double* U, double tau, int use_work, uint8_t* bad_out);
+hipError_t launch_admissible_pending(Ctx& c, const double* U, double tau);
+hipError_t launch_init_prims(Ctx& c, double* U, double tau0, const double* const* prims, unsigned long long* nbad);
+hipError_t launch_fill(Ctx& c, double* p, double v, int64_t n);
+hipError_t launch_history(Ctx& c);
+hipError_t launch_reduce_diag(Ctx& c, const double* U, int what);
+hipError_t launch_project_shear(Ctx& c, double* U, int correct);
+hipError_t launch_relax(Ctx& c, double* U, double tau, double dtau);
+hipError_t launch_layout(Ctx& c, const double* aos_dev, double* soa, int to_soa);
+
+}  // namespace fivo

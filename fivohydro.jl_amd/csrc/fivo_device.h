@@ -1,0 +1,591 @@
+// =============================================================================
+// fivo_device.h -- per-cell FP64 physics of the 2+1D hot path, as device code.
+//
+// Restates, for one thread per cell or face, the reference's per-cell functions:
+// the EOS (src/eos.jl:39-84, 318-519; src2d/primrec2d.jl:115-234), the scaled
+// Bessel K (src2d/bessel2d.jl:79-115, src/eos.jl:27-34), the primitive recovery
+// (src2d/primrec2d.jl:33-742), the forward map, wave speeds and HLLE flux
+// (src2d/primrec2d.jl:753-796, src2d/fluxes2d.jl:24-177) and the MC limiter
+// (src/utils.jl:14-22). Julia semantics are kept where they change results:
+// NaN-propagating min/max, the NaN-passing clamp, sign(0) = 0.
+//
+// Design for gfx950: everything is FP64 VALU; the light-sector EOS evaluates
+// P, dP/dT and d2P/dT2 from one exp; the Newton keeps the T-only part of the
+// EOS of the current iterate in registers (the reference's bit-neutral memo,
+// src2d/primrec2d.jl:115-206), so 6 of 8 Jacobian columns cost no exp of the
+// light sector; the 4x4 solve is fully unrolled so nothing spills to scratch.
+// =============================================================================
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fd {
+
+constexpr double T_MIN = 1e-20;
+constexpr double T_SOLVE_MAX = 10.0;
+constexpr double PHI_CAP = 10000.0;
+constexpr double E_VAC = 1e-20;
+constexpr double E_FLOOR = 1e-20;
+constexpr double CHI_SRE = 0.99999999;
+constexpr double TINY = 1e-300;
+constexpr double EPS_POS = 1e-50;
+constexpr double U_CAP = 1.0e4;
+constexpr double SR_MARGIN = 0.98;
+constexpr double ASYM_X = 1.0e4;
+constexpr double PI_ = 3.141592653589793;
+
+enum : int { PRR_UNSET = 0, PRR_VACUUM = 1, PRR_CONVERGED = 2, PRR_NONFINITE_INITIAL = 3,
+             PRR_NONFINITE_JACOBIAN = 4, PRR_LINSOLVE_FAILED = 5, PRR_LINESEARCH_FAILED = 6,
+             PRR_RESIDUAL_TOO_LARGE = 8, PRR_MAXIT = 9 };
+
+// ---- Julia Base semantics ----------------------------------------------------
+__device__ __forceinline__ bool isf(double x) { return __builtin_isfinite(x); }
+__device__ __forceinline__ bool isnan_(double x) { return x != x; }
+__device__ __forceinline__ bool sgn(double x) { return __double_as_longlong(x) < 0; }
+__device__ __forceinline__ double jmax(double x, double y) {
+    bool py = (y > x) || (!sgn(y) && sgn(x));
+    return py ? (isnan_(x) ? x : y) : (isnan_(y) ? y : x);
+}
+__device__ __forceinline__ double jmin(double x, double y) {
+    bool py = (y < x) || (sgn(y) && !sgn(x));
+    return py ? (isnan_(x) ? x : y) : (isnan_(y) ? y : x);
+}
+__device__ __forceinline__ double jclamp(double x, double a, double b) { return x < a ? a : (x > b ? b : x); }
+__device__ __forceinline__ double jsign(double x) { return x > 0 ? 1.0 : (x < 0 ? -1.0 : x); }
+__device__ __forceinline__ double posden(double x) { return x > EPS_POS ? x : EPS_POS; }
+__device__ __forceinline__ double safe_inv(double x) { return 1.0 / posden(x); }
+__device__ __forceinline__ double safe_div(double a, double b) { return a / posden(b); }
+__device__ __forceinline__ double safe_sqrt(double x) { return x <= 0 ? 0.0 : sqrt(x); }
+__device__ __forceinline__ double pow4(double x) { double x2 = x * x; return x2 * x2; }
+__device__ __forceinline__ double pow3(double x) { return x * x * x; }
+__device__ __forceinline__ double dinf() { return __longlong_as_double(0x7ff0000000000000LL); }
+
+// mc_limiter (src/utils.jl:14-22)
+__device__ __forceinline__ double mc_limiter(double a, double b) {
+    double sa = jsign(a), sb = jsign(b);
+    if (sa != sb) return 0.0;
+    double aa = fabs(a), bb = fabs(b);
+    return sa * jmin(2 * aa, jmin(2 * bb, 0.5 * (aa + bb)));
+}
+
+// ---- Bessel K ----------------------------------------------------------------
+// e^x K0(x), e^x K1(x): Boost's rational minimax approximations (coefficients by
+// P. Holoborodko), the table the reference uses (src2d/bessel2d.jl:37-103).
+// evalpoly: p0 + x*(p1 + x*(... + x*pn)) with one rounding per step (fma)
+__device__ __forceinline__ double hp(double, double c0) { return c0; }
+template <typename... R>
+__device__ __forceinline__ double hp(double x, double c0, R... r) { return fma(x, hp(x, r...), c0); }
+#define FD_H(x, ...) hp(x, __VA_ARGS__)
+__device__ __forceinline__ double k0x_small_P1(double a) { return FD_H(a, -1.372509002685546267e-1, 2.574916117833312855e-1, 1.395474602146869316e-2, 5.445476986653926759e-4, 7.125159422136622118e-6); }
+__device__ __forceinline__ double k0x_small_Q1(double a) { return FD_H(a, 1.0, -5.458333438017788530e-02, 1.291052816975251298e-03, -1.367653946978586591e-05); }
+__device__ __forceinline__ double k0x_small_P2(double a) { return FD_H(a, 1.159315156584124484e-01, 2.789828789146031732e-01, 2.524892993216121934e-02, 8.460350907213637784e-04, 1.491471924309617534e-05, 1.627106892422088488e-07, 1.208266102392756055e-09, 6.611686391749704310e-12); }
+__device__ __forceinline__ double k0x_big_P3(double a) { return FD_H(a, 2.533141373155002416e-1, 3.628342133984595192e0, 1.868441889406606057e1, 4.306243981063412784e1, 4.424116209627428189e1, 1.562095339356220468e1, -1.810138978229410898e0, -1.414237994269995877e0, -9.369168119754924625e-2); }
+__device__ __forceinline__ double k0x_big_Q3(double a) { return FD_H(a, 1.0, 1.494194694879908328e1, 8.265296455388554217e1, 2.162779506621866970e2, 2.845145155184222157e2, 1.851714491916334995e2, 5.486540717439723515e1, 6.118075837628957015e0, 1.586261269326235053e-1); }
+__device__ __forceinline__ double k1x_small_P1(double a) { return FD_H(a, -3.62137953440350228e-3, 7.11842087490330300e-3, 1.00302560256614306e-5, 1.77231085381040811e-6); }
+__device__ __forceinline__ double k1x_small_Q1(double a) { return FD_H(a, 1.0, -4.80414794429043831e-2, 9.85972641934416525e-4, -8.91196859397070326e-6); }
+__device__ __forceinline__ double k1x_small_P2(double a) { return FD_H(a, -3.07965757829206184e-1, -7.80929703673074907e-02, -2.70619343754051620e-3, -2.49549522229072008e-5); }
+__device__ __forceinline__ double k1x_small_Q2(double a) { return FD_H(a, 1.0, -2.36316836412163098e-2, 2.64524577525962719e-4, -1.49749618004162787e-6); }
+__device__ __forceinline__ double k1x_big_P3(double a) { return FD_H(a, -1.97028041029226295e-1, -2.32408961548087617e0, -7.98269784507699938e0, -2.39968410774221632e0, 3.28314043780858713e1, 5.67713761158496058e1, 3.30907788466509823e1, 6.62582288933739787e0, 3.08851840645286691e-1); }
+__device__ __forceinline__ double k1x_big_Q3(double a) { return FD_H(a, 1.0, 1.41811409298826118e1, 7.35979466317556420e1, 1.77821793937080859e2, 2.11014501598705982e2, 1.19425262951064454e2, 2.88448064302447607e1, 2.27912927104139732e0, 2.50358186953478678e-2); }
+#undef FD_H
+
+// e^x K0 and e^x K1 together: the large-x branch shares 1/x and sqrt(x),
+// the small-x branch shares log(x) and exp(x).
+__device__ __forceinline__ void besselk01x(double x, double* k0, double* k1) {
+    if (x <= 1.0) {
+        double lx = log(x), ex = exp(x), z = x * x, a = z / 4;
+        double s = fma(k0x_small_P1(a), 1.0 / k0x_small_Q1(a), 1.137250900268554688);
+        double a0 = fma(s, a, 1.0);
+        *k0 = fma(-a0, lx, k0x_small_P2(z)) * ex;
+        double pq = fma(k1x_small_P1(a), 1.0 / k1x_small_Q1(a), 8.69547128677368164e-2);
+        pq = fma(pq * a, a, (a / 2 + 1));
+        double a1 = pq * x / 2;
+        double pq2 = fma(k1x_small_P2(z) / k1x_small_Q2(z), x, 1.0 / x);
+        *k1 = fma(a1, lx, pq2) * ex;
+    } else {
+        double ix = 1.0 / x, sx = sqrt(x);
+        *k0 = fma(k0x_big_P3(ix), 1.0 / k0x_big_Q3(ix), 1.0) / sx;
+        *k1 = fma(k1x_big_P3(ix), 1.0 / k1x_big_Q3(ix), 1.45034217834472656) / sx;
+    }
+}
+// large-x series of safe_besselkx (src/eos.jl:27-34)
+__device__ __forceinline__ double besselkx_asym(int nu, double x) {
+    double mu = 4.0 * double(nu) * double(nu);
+    double ix = 1.0 / (8.0 * x);
+    double s = 1.0 + (mu - 1.0) * ix * (1.0 + (mu - 9.0) * ix / 2.0 * (1.0 + (mu - 25.0) * ix / 3.0));
+    return sqrt(PI_ / (2.0 * x)) * s;
+}
+// safe_besselkx(nu, x), nu = 0..5: K0, K1 and the upward recurrence
+// K_{n+1} = K_{n-1} + (2n/x) K_n (Abramowitz & Stegun 9.6.26), stable for K.
+__device__ __forceinline__ double safe_besselkx(int nu, double x) {
+    if (!(isf(x) && x > 0.0)) return 0.0;
+    if (x >= ASYM_X) return besselkx_asym(nu, x);
+    double km, k;
+    besselk01x(x, &km, &k);
+    if (nu == 0) return km;
+    for (int n = 1; n < nu; ++n) { double kp = km + (2.0 * n / x) * k; km = k; k = kp; }
+    return k;
+}
+// safe_besselk2x_2d (src2d/bessel2d.jl:111-115)
+__device__ __forceinline__ double besselk2x_fast(double x) {
+    if (!(isf(x) && x > 0.0)) return 0.0;
+    if (x >= ASYM_X) return besselkx_asym(2, x);
+    double k0, k1;
+    besselk01x(x, &k0, &k1);
+    return k0 + (2 / x) * k1;
+}
+
+// ---- EOS constants (host-prepared, same arithmetic as the reference) ----------
+struct EosP {
+    int kind;            // 0 conformal, 1 lattice-HRG
+    int has_charge;      // eos_has_charge_2d
+    int hq_fm;
+    int conf_charm;      // conformal with g_hq > 0: charm sector evaluated
+    double m;            // hq_mass
+    double aSB;          // (pi^2/90) g_eff
+    double Aconf;        // g m^2 / (2 pi^2)
+    double Alhrg;        // canon g m^2 / (2 pi^2)
+    double fmGeV3;
+    // lattice light sector: A = 0.01 d^2, B = 0.1 c^2 and the products of the
+    // fit coefficients with the k-constants in src/eos.jl:357-460
+    double LA, LB, nA4, nA3, nA2, nA1, nA0, dB4, dB3, dB2, dB1;
+    double nP1, nP2, nP3, nP4, dP1, dP2, dP3;   // first-derivative coefficients
+    double nQ2, nQ3, nQ4, dQ2, dQ3;             // second-derivative coefficients
+};
+
+// light_P, light_dP_dT, light_d2P_dT2 (src/eos.jl:357-460) from one exp. T >= T_MIN.
+template <int ORDER>
+__device__ __forceinline__ void light_eval(double T, const EosP& q, double* P, double* dP, double* d2P) {
+    double T2 = T * T, T3 = pow3(T), T4 = pow4(T);
+    double E = -(q.LA / T2) - (q.LB / T);
+    E = jclamp(E, -700.0, 700.0);
+    double expE = exp(E);
+    double N = q.nA0 + q.nA1 * T + q.nA2 * T2 + q.nA3 * T3 + q.nA4 * T4;
+    double D = q.dB4 + q.dB3 * T + q.dB2 * T2 + q.dB1 * T3 + T4;
+    double ND = N / D;
+    *P = q.fmGeV3 * expE * T4 * ND;
+    if (ORDER < 1) return;
+    double Ep = (2 * q.LA / T3) + (q.LB / T2);
+    double Np = q.nP1 + q.nP2 * T + q.nP3 * T2 + q.nP4 * T3;
+    double Dp = q.dP1 + q.dP2 * T + q.dP3 * T2 + 4 * T3;
+    double G = T4 * N, H = D;
+    double Gp = 4 * T3 * N + T4 * Np, Hp = Dp;
+    double F = G / H;
+    double Q = (Gp * H - G * Hp);
+    double Fp = Q / (H * H);
+    *dP = q.fmGeV3 * expE * (Fp + F * Ep);
+    if (ORDER < 2) return;
+    double Epp = (-6 * q.LA / T4) - (2 * q.LB / T3);
+    double Npp = q.nQ2 + q.nQ3 * T + q.nQ4 * T2;
+    double Dpp = q.dQ2 + q.dQ3 * T + 12 * T2;
+    double Gpp = 12 * T2 * N + 8 * T3 * Np + T4 * Npp, Hpp = Dpp;
+    double Qp = (Gpp * H - G * Hpp);
+    double Fpp = (Qp * H - 2 * Q * Hp) / (H * H * H);
+    *d2P = q.fmGeV3 * expE * (Fpp + 2 * Fp * Ep + F * (Epp + Ep * Ep));
+}
+
+// T-only part of the EOS at T (>= T_MIN): what the reference memoises per T
+struct TPart { double T, Pl, el, K2, K1, x; };
+
+template <int EK>
+__device__ __forceinline__ TPart eos_tpart(double T, const EosP& q) {
+    TPart t;
+    t.T = T;
+    if (EK == 0) {
+        t.Pl = q.aSB * pow4(T) * q.fmGeV3;
+        t.el = 3.0 * t.Pl;
+        t.x = q.m / T;
+        t.K2 = 0.0; t.K1 = 0.0;
+        if (q.conf_charm) { t.K2 = safe_besselkx(2, t.x); t.K1 = safe_besselkx(1, t.x); }
+    } else {
+        double P, dP, d2P;
+        light_eval<1>(T, q, &P, &dP, &d2P);
+        t.Pl = P;
+        t.el = -P + T * dP;
+        t.x = jmin(q.m / T, 100000.0);
+        t.K2 = (!isf(t.x) || t.x <= 0.0) ? 0.0 : besselk2x_fast(t.x);
+        t.K1 = 0.0;
+    }
+    return t;
+}
+// assemble (P, n, e) from the T-part and mu: eos_Pne (src/eos.jl:51-84) for the
+// conformal EOS, _lhrg_assemble_2d (src2d/primrec2d.jl:147-166) for lattice-HRG.
+template <int EK>
+__device__ __forceinline__ void eos_assemble(const TPart& t, double mu, const EosP& q, double* P, double* n,
+                                             double* e) {
+    const double T = t.T;
+    if (EK == 0) {
+        if (!q.conf_charm || !isf(t.x) || t.x <= 0.0) { *P = t.Pl; *n = 0.0; *e = t.el; return; }
+        double Efac = exp((mu - q.m) / T);
+        double nh = q.Aconf * T * Efac * t.K2;
+        double Ph = q.Aconf * (T * T) * Efac * t.K2;
+        double eh = q.Aconf * (T * T) * Efac * (3.0 * t.K2 + t.x * t.K1);
+        if (q.hq_fm) { nh *= q.fmGeV3; Ph *= q.fmGeV3; eh *= q.fmGeV3; }
+        *P = t.Pl + Ph; *n = nh; *e = t.el + eh;
+    } else {
+        if (!isf(t.x) || t.x <= 0.0) { *P = t.Pl; *n = 0.0; *e = t.el; return; }
+        double z = jclamp((mu - q.m) / T, -700.0, 700.0);
+        double nh = q.Alhrg * T * exp(z) * t.K2;
+        if (q.hq_fm) nh *= q.fmGeV3;
+        *P = t.Pl + 0.0; *n = nh; *e = t.el + 0.0;
+    }
+}
+template <int EK>
+__device__ __forceinline__ void eos_Pne_2d(double T, double mu, const EosP& q, double* P, double* n, double* e) {
+    TPart t = eos_tpart<EK>(jmax(T, T_MIN), q);
+    eos_assemble<EK>(t, mu, q, P, n, e);
+}
+// eos_Pe_2d (src2d/primrec2d.jl:218-234): P and e for the Newton seed
+template <int EK>
+__device__ __forceinline__ void eos_Pe_2d(double T, double mu, const EosP& q, double* P, double* e) {
+    T = jmax(T, T_MIN);
+    if (EK == 0) {
+        double n;
+        TPart t = eos_tpart<0>(T, q);
+        eos_assemble<0>(t, mu, q, P, &n, e);
+    } else {
+        double Pl, dP, d2P;
+        light_eval<1>(T, q, &Pl, &dP, &d2P);
+        *P = Pl + 0.0;
+        *e = (-Pl + T * dP) + 0.0;
+    }
+}
+// eos_cs2 (src/eos.jl:49, 471-476)
+template <int EK>
+__device__ __forceinline__ double eos_cs2(double T, const EosP& q) {
+    if (EK == 0) return 1.0 / 3.0;
+    T = jmax(T, T_MIN);
+    double P, dP, d2P;
+    light_eval<2>(T, q, &P, &dP, &d2P);
+    return (d2P == 0.0) ? (1.0 / 3.0) : (dP / (T * d2P));
+}
+
+// ---- shear algebra (src2d/shear2d.jl:61-181) ---------------------------------
+__device__ __forceinline__ void shear_tau_row(double ux, double uy, double ut, double pixx, double pixy,
+                                              double piyy, double* tt, double* tx, double* ty) {
+    double iu = safe_inv(ut);
+    *tx = (ux * pixx + uy * pixy) * iu;
+    *ty = (ux * pixy + uy * piyy) * iu;
+    *tt = (ux * *tx + uy * *ty) * iu;
+}
+__device__ __forceinline__ double nu_tau(double ux, double uy, double ut, double nux, double nuy) {
+    return (ux * nux + uy * nuy) * safe_inv(ut);
+}
+
+// dissipative dofs of one cell in physical units (dissipatives_at)
+struct Diss { double nux, nuy, Pi, pixx, pixy, piyy, pieta; };
+
+// ---- primitive recovery (src2d/primrec2d.jl:300-742) --------------------------
+struct PrimRes { double T, mu, ux, uy, n, e, P; int ok, reason, iters; };
+
+template <int EK>
+__device__ __forceinline__ double invert_e_for_T(double e_target, double phi, const EosP& q) {
+    double lo = log(T_MIN), hi = log(T_SOLVE_MAX);
+    if (e_target <= 0.0) return lo;
+    for (int it = 0; it < 60; ++it) {
+        double mid = 0.5 * (lo + hi);
+        double Tm = exp(mid), P, em;
+        eos_Pe_2d<EK>(Tm, q.m + Tm * phi, q, &P, &em);
+        if (!isf(em) || em > e_target) hi = mid; else lo = mid;
+    }
+    return 0.5 * (lo + hi);
+}
+template <int EK>
+__device__ __noinline__ void seed_from_conserved(double E, double Sx, double Sy, double phi, const EosP& q,
+                                                 double* yT_o, double* ux_o, double* uy_o) {
+    double M = sqrt(Sx * Sx + Sy * Sy);
+    if (!isf(E) || E <= 0.0) { *yT_o = log(0.15); *ux_o = 0.0; *uy_o = 0.0; return; }
+    double P = E / 3, yT = log(0.15), v = 0.0;
+    for (int it = 0; it < 3; ++it) {
+        double denom = E + P;
+        v = denom > TINY ? jmin(M / denom, 0.999999) : 0.0;
+        double e = denom * (1 - v * v) - P;
+        if (e <= 0.0) e = jmax(E, TINY);
+        yT = invert_e_for_T<EK>(e, phi, q);
+        double T = exp(yT), Pn, en;
+        eos_Pe_2d<EK>(T, q.m + T * phi, q, &Pn, &en);
+        if (!isf(Pn)) break;
+        P = Pn;
+    }
+    double gam = 1 / sqrt(jmax(1 - v * v, 1e-16));
+    double us = gam * v;
+    double ux = M > TINY ? us * Sx / M : 0.0;
+    double uy = M > TINY ? us * Sy / M : 0.0;
+    *yT_o = yT;
+    *ux_o = jclamp(ux, -U_CAP, U_CAP);
+    *uy_o = jclamp(uy, -U_CAP, U_CAP);
+}
+
+// evalF_2d! (src2d/primrec2d.jl:360-390), with the EOS T-part of T = exp(x0)
+// given. Writes F (all Inf when the state or the EOS is non-finite, like the
+// reference) and returns whether every component is finite.
+template <int EK, bool NCH>
+__device__ __forceinline__ bool evalF(double F[4], double x0, double x1, double x2, double x3, const TPart& tp,
+                                      double D, double Sx, double Sy, double E, const Diss& d, const EosP& q) {
+    if (!(isf(x0) && isf(x1) && isf(x2) && isf(x3))) { F[0] = F[1] = F[2] = F[3] = dinf(); return false; }
+    double T = exp(x0);
+    double mu = q.m + T * x1;
+    double ut = sqrt(1 + x2 * x2 + x3 * x3);
+    double P, n, e;
+    eos_assemble<EK>(tp, mu, q, &P, &n, &e);
+    if (!(isf(P) && isf(n) && isf(e))) { F[0] = F[1] = F[2] = F[3] = dinf(); return false; }
+    double weff = e + P + d.Pi, Ptot = P + d.Pi;
+    double tt, tx, ty;
+    shear_tau_row(x2, x3, ut, d.pixx, d.pixy, d.piyy, &tt, &tx, &ty);
+    F[0] = NCH ? (n * ut + nu_tau(x2, x3, ut, d.nux, d.nuy) - D) : 0.0;
+    F[1] = weff * ut * x2 + tx - Sx;
+    F[2] = weff * ut * x3 + ty - Sy;
+    F[3] = weff * ut * ut - Ptot + tt - E;
+    return isf(F[0]) && isf(F[1]) && isf(F[2]) && isf(F[3]);
+}
+template <int EK>
+__device__ __forceinline__ TPart tpart_of_yT(double yT, const EosP& q) {
+    return eos_tpart<EK>(jmax(exp(yT), T_MIN), q);
+}
+
+// solve_gauss_n! (src2d/primrec2d.jl:33-87), n = N, fully unrolled on registers
+template <int N>
+__device__ __forceinline__ bool solve_gauss(double J[4][4], const double F[4], double delta[4]) {
+    double A[N][N + 1];
+#pragma unroll
+    for (int r = 0; r < N; ++r) {
+#pragma unroll
+        for (int c = 0; c < N; ++c) A[r][c] = J[r][c];
+        A[r][N] = -F[r];
+    }
+    double smax = 0.0;
+#pragma unroll
+    for (int r = 0; r < N; ++r)
+#pragma unroll
+        for (int c = 0; c < N; ++c) smax = jmax(smax, fabs(A[r][c]));
+    if (!isf(smax) || smax == 0.0) return false;
+    const double pivtol = 1e-15 * smax;
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        // partial pivoting: select-based swap keeps the matrix in registers
+        double amax = fabs(A[k][k]);
+        int p = k;
+#pragma unroll
+        for (int r = k + 1; r < N; ++r) if (fabs(A[r][k]) > amax) { amax = fabs(A[r][k]); p = r; }
+        if (amax < pivtol) return false;
+#pragma unroll
+        for (int r = k + 1; r < N; ++r) {
+            bool sw = (p == r);
+#pragma unroll
+            for (int c = k; c <= N; ++c) {
+                double a = A[k][c], b = A[r][c];
+                A[k][c] = sw ? b : a;
+                A[r][c] = sw ? a : b;
+            }
+        }
+        double invp = 1.0 / A[k][k];
+#pragma unroll
+        for (int r = k + 1; r < N; ++r) {
+            double f = A[r][k] * invp;
+            if (f != 0.0) {
+#pragma unroll
+                for (int c = k; c <= N; ++c) A[r][c] -= f * A[k][c];
+            }
+        }
+    }
+#pragma unroll
+    for (int r = N - 1; r >= 0; --r) {
+        double s = A[r][N];
+#pragma unroll
+        for (int c = r + 1; c < N; ++c) s -= A[r][c] * delta[c];
+        delta[r] = s / A[r][r];
+    }
+    bool fin = true;
+#pragma unroll
+    for (int r = 0; r < N; ++r) fin = fin && isf(delta[r]);
+    return fin;
+}
+
+// _solve_phi_bisect (src2d/primrec2d.jl:416-434) at the (clamped) final T
+template <int EK>
+__device__ __noinline__ bool solve_phi_bisect(double D, double ux, double uy, double nux, double nuy,
+                                              const TPart& tp, const EosP& q, double* phi_o) {
+    double ut = sqrt(1 + ux * ux + uy * uy);
+    double target = D - nu_tau(ux, uy, ut, nux, nuy);
+    if (!isf(target) || target <= 0.0) { *phi_o = -PHI_CAP; return false; }
+    double lo = -PHI_CAP, hi = PHI_CAP, P, nm, e;
+    for (int it = 0; it < 200; ++it) {
+        double mid = 0.5 * (lo + hi);
+        eos_assemble<EK>(tp, q.m + tp.T * mid, q, &P, &nm, &e);
+        if (isf(nm) && nm * ut > target) hi = mid; else lo = mid;
+    }
+    double phi = 0.5 * (lo + hi);
+    eos_assemble<EK>(tp, q.m + tp.T * phi, q, &P, &nm, &e);
+    *phi_o = phi;
+    return isf(nm);
+}
+
+// _cons_to_prim_attempt_2d! (src2d/primrec2d.jl:511-742). NCH: eos_has_charge_2d.
+template <int EK, bool NCH>
+__device__ __forceinline__ PrimRes cons_to_prim_attempt(double D, double Sx, double Sy, double E, const Diss& d,
+                                                     double yT0, double phi0, double ux0, double uy0,
+                                                     const EosP& q) {
+    constexpr int NU = NCH ? 4 : 3;
+    constexpr double tol = 1e-12;
+    constexpr int maxit = 80;
+    const double yT_lo = log(T_MIN), yT_hi = log(T_SOLVE_MAX);
+    PrimRes R;
+    R.reason = PRR_UNSET; R.iters = 0;
+    if (!isf(E) || E <= E_VAC || !isf(D) || D < 0.0) {
+        double P, n, e;
+        eos_Pne_2d<EK>(T_MIN, 0.0, q, &P, &n, &e);   // eos_Pne at T_MIN: identical branch
+        R = {T_MIN, 0.0, 0.0, 0.0, 0.0, jmax(E, 0.0), P, 0, PRR_VACUUM, 0};
+        return R;
+    }
+    double x[4];
+    x[1] = (NCH && isf(phi0)) ? jclamp(phi0, -PHI_CAP, PHI_CAP) : 0.0;
+    if (isf(yT0) && isf(ux0) && isf(uy0)) {
+        x[0] = jclamp(yT0, yT_lo, yT_hi);
+        x[2] = jclamp(ux0, -U_CAP, U_CAP);
+        x[3] = jclamp(uy0, -U_CAP, U_CAP);
+    } else {
+        double sy, sx, syy;
+        seed_from_conserved<EK>(E, Sx, Sy, x[1], q, &sy, &sx, &syy);
+        x[0] = jclamp(sy, yT_lo, yT_hi); x[2] = sx; x[3] = syy;
+    }
+    TPart tp = tpart_of_yT<EK>(x[0], q);
+    double F[4];
+    if (!evalF<EK, NCH>(F, x[0], x[1], x[2], x[3], tp, D, Sx, Sy, E, d, q)) {
+        double T = jclamp(exp(x[0]), T_MIN, T_SOLVE_MAX), P, n, e;
+        eos_Pne_2d<EK>(T, q.m, q, &P, &n, &e);
+        R = {T, q.m, 0.0, 0.0, 0.0, jmax(e, 0.0), P, 0, PRR_NONFINITE_INITIAL, 0};
+        return R;
+    }
+    const double sD = jmax(fabs(D), 1e-12);
+    const double sS = jmax(jmax(jmax(fabs(Sx), fabs(Sy)), 1e-10 * fabs(E)), 1e-12);
+    const double sE = jmax(fabs(E), 1e-12);
+    const double rs[4] = {1 / sD, 1 / sS, 1 / sS, 1 / sE};
+    constexpr int rlo = NCH ? 0 : 1;
+    bool converged = false;
+    int reason = PRR_UNSET, iters = 0;
+    for (int it = 1; it <= maxit; ++it) {
+        double nrm = 0.0;
+#pragma unroll
+        for (int r = rlo; r < 4; ++r) nrm = jmax(nrm, fabs(F[r] * rs[r]));
+        if (nrm < tol) { converged = true; reason = PRR_CONVERGED; iters = it - 1; break; }
+        double J[4][4];
+        bool okj = true;
+#pragma unroll
+        for (int k = 0; k < NU; ++k) {
+            const int s = NCH ? k : (k == 0 ? 0 : k + 1);
+            double xs = x[s];
+            double h = 1e-6 * jmax(1.0, fabs(xs));
+            double Fp[4], Fm[4];
+            double xp[4] = {x[0], x[1], x[2], x[3]}, xm[4] = {x[0], x[1], x[2], x[3]};
+            xp[s] = xs + h; xm[s] = xs - h;
+            bool a, b;
+            if (s == 0) {
+                TPart tpp = tpart_of_yT<EK>(xp[0], q);
+                TPart tpm = tpart_of_yT<EK>(xm[0], q);
+                a = evalF<EK, NCH>(Fp, xp[0], xp[1], xp[2], xp[3], tpp, D, Sx, Sy, E, d, q);
+                b = evalF<EK, NCH>(Fm, xm[0], xm[1], xm[2], xm[3], tpm, D, Sx, Sy, E, d, q);
+            } else {
+                a = evalF<EK, NCH>(Fp, xp[0], xp[1], xp[2], xp[3], tp, D, Sx, Sy, E, d, q);
+                b = evalF<EK, NCH>(Fm, xm[0], xm[1], xm[2], xm[3], tp, D, Sx, Sy, E, d, q);
+            }
+            if (!(a && b)) { okj = false; break; }
+            double inv2h = 1 / (2 * h);
+#pragma unroll
+            for (int r = 0; r < NU; ++r) {
+                const int rr = NCH ? r : r + 1;
+                J[r][k] = ((Fp[rr] - Fm[rr]) * inv2h) * rs[rr];
+            }
+        }
+        if (!okj) { reason = PRR_NONFINITE_JACOBIAN; break; }
+        double Fact[4], delta[4];
+#pragma unroll
+        for (int r = 0; r < NU; ++r) {
+            const int rr = NCH ? r : r + 1;
+            Fact[r] = F[rr] * rs[rr];
+            delta[r] = 0.0;
+        }
+        if (!solve_gauss<NU>(J, Fact, delta)) { reason = PRR_LINSOLVE_FAILED; break; }
+        double alpha = 1.0;
+#pragma unroll
+        for (int k = 0; k < NU; ++k) { double ad = fabs(delta[k]); if (ad > 1.0) alpha = jmin(alpha, 1.0 / ad); }
+        const double xs0 = x[0], xs1 = x[1], xs2 = x[2], xs3 = x[3];
+        bool accepted = false;
+        for (int ls = 0; ls < 12; ++ls) {
+            double t[4] = {xs0, xs1, xs2, xs3};
+#pragma unroll
+            for (int k = 0; k < NU; ++k) {
+                const int s = NCH ? k : (k == 0 ? 0 : k + 1);
+                t[s] += alpha * delta[k];
+            }
+            t[0] = jclamp(t[0], yT_lo, yT_hi);
+            if (NCH) t[1] = jclamp(t[1], -PHI_CAP, PHI_CAP);
+            t[2] = jclamp(t[2], -U_CAP, U_CAP);
+            t[3] = jclamp(t[3], -U_CAP, U_CAP);
+            x[0] = t[0]; x[1] = t[1]; x[2] = t[2]; x[3] = t[3];
+            double Tn = jmax(exp(t[0]), T_MIN);
+            if (Tn != tp.T) tp = eos_tpart<EK>(Tn, q);
+            if (evalF<EK, NCH>(F, t[0], t[1], t[2], t[3], tp, D, Sx, Sy, E, d, q)) {
+                double trial = 0.0;
+#pragma unroll
+                for (int r = rlo; r < 4; ++r) trial = jmax(trial, fabs(F[r] * rs[r]));
+                if (trial < nrm * (1 - 1e-4 * alpha)) { accepted = true; break; }
+            }
+            alpha *= 0.5;
+        }
+        if (!accepted) { reason = PRR_LINESEARCH_FAILED; break; }
+        if (it == maxit) iters = maxit;
+    }
+    double T = jclamp(exp(x[0]), T_MIN, T_SOLVE_MAX);
+    double mu = NCH ? (q.m + T * x[1]) : q.m;
+    double ux = x[2], uy = x[3];
+    TPart tf = (jmax(T, T_MIN) == tp.T) ? tp : eos_tpart<EK>(jmax(T, T_MIN), q);
+    double P, n, e;
+    eos_assemble<EK>(tf, mu, q, &P, &n, &e);
+    if (!converged) reason = (reason == PRR_UNSET) ? PRR_MAXIT : reason;
+    // residual at the final iterate (tp belongs to x[0])
+    double rows[4] = {0, 0, 0, 0};
+    double resnorm = 0.0;
+    evalF<EK, NCH>(F, x[0], x[1], x[2], x[3], tp, D, Sx, Sy, E, d, q);
+#pragma unroll
+    for (int r = rlo; r < 4; ++r) { rows[r] = fabs(F[r] * rs[r]); resnorm = jmax(resnorm, rows[r]); }
+    R.iters = iters;
+    if (!(isf(resnorm) && resnorm < tol)) {
+        double hyd = jmax(jmax(rows[1], rows[2]), rows[3]);
+        if (NCH && isf(hyd) && hyd < tol) {
+            double phib;
+            bool okb = solve_phi_bisect<EK>(D, ux, uy, d.nux, d.nuy, tf, q, &phib);
+            double ph = jclamp(phib, -PHI_CAP, PHI_CAP);
+            mu = q.m + T * ph;
+            eos_assemble<EK>(tf, mu, q, &P, &n, &e);
+            R = {T, mu, ux, uy, jmax(n, 0.0), jmax(e, 0.0), P, 1, okb ? PRR_CONVERGED : PRR_RESIDUAL_TOO_LARGE, iters};
+            return R;
+        }
+        R = {T, mu, ux, uy, n, jmax(e, 0.0), P, 0, PRR_RESIDUAL_TOO_LARGE, iters};
+        return R;
+    }
+    R = {T, mu, ux, uy, n, jmax(e, 0.0), P, 1, PRR_CONVERGED, iters};
+    (void)reason;
+    return R;
+}
+
+// cons_to_prim_2d! (src2d/primrec2d.jl:465-486): one fresh-seed retry. Written
+// as a two-pass loop so the Newton body is instantiated once.
+template <int EK, bool NCH>
+__device__ __forceinline__ PrimRes cons_to_prim(double D, double Sx, double Sy, double E, const Diss& d,
+                                                double yT0, double phi0, double ux0, double uy0, const EosP& q) {
+    PrimRes r;
+    int its = 0;
+    const bool warm = isf(yT0) && isf(ux0) && isf(uy0);
+    const double nan = __longlong_as_double(0x7ff8000000000000LL);
+#pragma unroll 1
+    for (int pass = 0; pass < 2; ++pass) {
+        r = cons_to_prim_attempt<EK, NCH>(D, Sx, Sy, E, d, yT0, phi0, ux0, uy0, q);
+        its += r.iters;
+        if (r.ok || r.reason == PRR_VACUUM || !warm) break;
+        yT0 = nan; phi0 = nan; ux0 = nan; uy0 = nan;
+    }
+    r.iters = its;
+    return r;
+}
+
+}  // namespace fd

@@ -1,0 +1,106 @@
+"""GPU parity: the HIP library against the CPU oracle on identical inputs.
+
+Bar (from the task's north star): per-RHS relative difference <= 1e-12, final
+time relative difference <= 1e-9 in T and u. "Relative" is per field: the max
+abs difference over cells divided by the field's max abs value.
+"""
+import numpy as np
+import pytest
+
+import fivo2d as F
+from fivo2d import ic
+from helpers import gubser_problem, make_solver, rel_diff
+
+pytestmark = pytest.mark.gpu
+
+RHS_TOL = 1e-12
+FINAL_TOL = 1e-9
+
+
+def _pair(oracle, hip, g, m, prims, tau0):
+    return make_solver(oracle, g, m, prims, tau0), make_solver(hip, g, m, prims, tau0)
+
+
+def _check_rhs(so, sh, tau, force=None, tol=RHS_TOL, bc=F.FIVO_BC_OUTFLOW):
+    Uo, Uh = so.get_state(), sh.get_state()
+    assert np.max(rel_diff(Uh, Uo)) <= 1e-14
+    dUo, co = so.rhs_2d(tau, bc=bc, force_first_order=force)
+    dUh, ch = sh.rhs_2d(tau, bc=bc, force_first_order=force)
+    r = rel_diff(dUh, dUo)
+    assert np.all(np.isfinite(dUh))
+    assert np.max(r) <= tol, f"per-field dU rel diff {r}"
+    assert co["nprimfail"] == ch["nprimfail"] and co["nvacuum"] == ch["nvacuum"]
+    assert abs(co["amax"] - ch["amax"]) <= 1e-13 * co["amax"]
+    wo, wh = so.get_work(("yT", "ux", "uy", "P", "ok")), sh.get_work(("yT", "ux", "uy", "P", "ok"))
+    assert np.array_equal(wo["ok"], wh["ok"])
+    for k in ("yT", "ux", "uy", "P"):
+        assert np.max(np.abs(wh[k] - wo[k])) <= tol * max(np.max(np.abs(wo[k])), 1e-300), k
+    return r
+
+
+def test_rhs_parity_gubser_conformal(oracle, hip):
+    """Config C1 state (conformal e=3P, no charge): cold-seed pass, then warm."""
+    g, m, prims, _ = gubser_problem(N=128)
+    so, sh = _pair(oracle, hip, g, m, prims, 1.0)
+    _check_rhs(so, sh, 1.0)          # cold Newton (NaN warm starts)
+    _check_rhs(so, sh, 1.0)          # warm-started Newton
+
+
+def test_rhs_parity_gubser_charm_eos(oracle, hip):
+    """Gate-G1 EOS: ConformalHQEOS() with the charm Boltzmann sector (4 unknowns)."""
+    g, m, prims, _ = gubser_problem(N=100, g_hq=6.0, alpha=-20.0)
+    so, sh = _pair(oracle, hip, g, m, prims, 1.0)
+    _check_rhs(so, sh, 1.0)
+    _check_rhs(so, sh, 1.0)
+
+
+def test_rhs_parity_forced_first_order(oracle, hip):
+    """MOOD path: a force_first_order mask drops faces to piecewise constant."""
+    g, m, prims, _ = gubser_problem(N=64)
+    so, sh = _pair(oracle, hip, g, m, prims, 1.0)
+    rng = np.random.default_rng(7)
+    mask = np.zeros(g.Ntot, dtype=np.uint8)
+    X = mask.reshape(g.Nxtot, g.Nytot)
+    sx, sy = g.interior()
+    X[sx, sy] = rng.random((g.Nx, g.Ny)) < 0.2
+    _check_rhs(so, sh, 1.0, force=mask)
+
+
+def test_rhs_parity_periodic(oracle, hip):
+    g, m, prims, _ = gubser_problem(N=48)
+    so, sh = _pair(oracle, hip, g, m, prims, 1.0)
+    _check_rhs(so, sh, 1.0, bc=F.FIVO_BC_PERIODIC)
+
+
+def test_rhs_parity_lattice_hrg_uniform(oracle, hip):
+    """LatticeHRG EOS with the charm density (fast K2) on a Bjorken state."""
+    g = F.make_grid2d(24, 24, xmax=6.0, ymax=6.0)
+    m = F.build_model_2d(eos=F.LatticeHRGEOS())
+    prims = ic.uniform(g, 0.5, -4.2)
+    so, sh = _pair(oracle, hip, g, m, prims, 0.4)
+    _check_rhs(so, sh, 0.4)
+    _check_rhs(so, sh, 0.4)
+
+
+def _final_compare(so, sh, g):
+    fo, fh = so.fields_2d(), sh.fields_2d()
+    dT = np.max(np.abs(fh["T"] - fo["T"])) / np.max(np.abs(fo["T"]))
+    du = max(np.max(np.abs(fh["ux"] - fo["ux"])), np.max(np.abs(fh["uy"] - fo["uy"]))) / \
+        max(np.max(np.abs(fo["ux"])), np.max(np.abs(fo["uy"])))
+    return dT, du
+
+
+@pytest.mark.parametrize("integ", ["ssprk3", "ssprk2"])
+def test_run_parity_c1(oracle, hip, integ):
+    """Config C1 (Gubser, conformal, 128^2, CFLtau 0.4, transverse CFL 0.2) to tau = 3."""
+    g, m, prims, _ = gubser_problem(N=128)
+    so, sh = _pair(oracle, hip, g, m, prims, 1.0)
+    kw = dict(tau0=1.0, taufinal=3.0, CFL=0.2, CFLtau=0.4, integrator=integ)
+    ro = so.run_sim_2d(**kw)
+    rh = sh.run_sim_2d(**kw)
+    assert ro["ok"] and rh["ok"]
+    assert ro["nsteps"] == rh["nsteps"] and ro["nhalvings"] == rh["nhalvings"]
+    assert abs(ro["tau"] - rh["tau"]) <= 1e-14
+    dT, du = _final_compare(so, sh, g)
+    assert dT <= FINAL_TOL and du <= FINAL_TOL, (dT, du)
+    assert abs(rh["Q1"] - ro["Q1"]) <= 1e-12 * max(abs(ro["Q1"]), 1e-300) + 1e-300

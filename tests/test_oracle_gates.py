@@ -1,0 +1,118 @@
+"""The oracle against the reference's published gates (CPU only).
+
+G1 (test/test_gubser2d.jl:97-246, numbers README2D.md:275): ideal Gubser flow,
+ConformalHQEOS() with the charm tracer at alpha = -20, tau 1 -> 2, CFL 0.15,
+CFLtau 0.05, SSPRK2, box +-10 fm, errors inside r <= 3 fm. Published:
+L2(T) 3.29e-3 / 8.81e-4, L2(u) 7.73e-3 / 1.92e-3 at N = 100 / 200, orders
+T 1.90, u 2.01, x<->y at round-off.
+G0 (test/test_bjorken2d.jl:94-169): transversely uniform LatticeHRG state;
+tau*J^tau drift < 1e-13, u identically 0, E spread < 1e-13, E/T/n vs the exact
+invariants tau*s and tau*n < 5e-4, temporal order 2 (SSPRK2) and 3 (SSPRK3).
+"""
+import numpy as np
+import pytest
+
+import fivo2d as F
+from fivo2d import ic
+from oracle_unit import OracleUnit
+
+
+def run_gubser(lib, N, tf=2.0):
+    g = F.make_grid2d(N, N, xmax=10.0, ymax=10.0)
+    m = F.build_model_2d(eos=F.ConformalHQEOS())
+    s = F.Solver(g, m, lib=lib)
+    prims, Ts = ic.gubser_ic(g, 1.0, 1.0, 0.6, -20.0)
+    s.init_from_prims(1.0, **prims)
+    r = s.run_sim_2d(tau0=1.0, taufinal=tf, CFL=0.15, CFLtau=0.05)
+    assert r["ok"] and r["nprimfail"] == 0
+    s.update_primitives_2d(r["tau"])
+    f = s.fields_2d()
+    tau = r["tau"]
+    X, Y = np.meshgrid(f["x"], f["y"], indexing="ij")
+    R = np.hypot(X, Y)
+    msk = R <= 3.0
+    Ta = ic.gubser_temperature(tau, R, 1.0, Ts)
+    ur = ic.gubser_ur(tau, R, 1.0)
+    uxa = np.where(R > 0, ur * X / np.where(R > 0, R, 1), 0)
+    uya = np.where(R > 0, ur * Y / np.where(R > 0, R, 1), 0)
+    L2T = np.sqrt(((f["T"] - Ta) ** 2)[msk].sum() / (Ta ** 2)[msk].sum())
+    L2U = np.sqrt((((f["ux"] - uxa) ** 2 + (f["uy"] - uya) ** 2)[msk]).sum() /
+                  ((uxa ** 2 + uya ** 2)[msk]).sum())
+    asym = np.max(np.abs(f["T"] - f["T"].T) / np.maximum(f["T"], f["T"].T))
+    return L2T, L2U, asym
+
+
+def test_gubser_gate_g1(oracle):
+    a = run_gubser(oracle, 100)
+    b = run_gubser(oracle, 200)
+    # the published numbers, to the digits the reference prints
+    assert abs(a[0] - 3.29e-3) < 0.005e-3 and abs(a[1] - 7.73e-3) < 0.005e-3
+    assert abs(b[0] - 8.81e-4) < 0.005e-4 and abs(b[1] - 1.92e-3) < 0.006e-3
+    oT, oU = np.log2(a[0] / b[0]), np.log2(a[1] / b[1])
+    assert abs(oT - 1.90) < 0.006 and abs(oU - 2.01) < 0.006
+    assert a[2] < 1e-12 and b[2] < 1e-12
+
+
+def _bjorken_exact(ou, eos, T0, mu0, tau0, tau):
+    P0, n0, e0 = ou.eos_Pne(eos, T0, mu0)
+    star = (e0 + P0) / T0 * tau0 / tau
+    ntar = n0 * tau0 / tau
+    lo, hi = 1e-4, 5.0
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        P, n, e = ou.eos_Pne(eos, mid, mu0)
+        if (e + P) / mid > star:
+            hi = mid
+        else:
+            lo = mid
+    T = 0.5 * (lo + hi)
+    mlo, mhi = -40.0, 40.0
+    for _ in range(200):
+        mid = 0.5 * (mlo + mhi)
+        if ou.eos_Pne(eos, T, mid)[1] > ntar:
+            mhi = mid
+        else:
+            mlo = mid
+    mu = 0.5 * (mlo + mhi)
+    P, n, e = ou.eos_Pne(eos, T, mu)
+    return T, n, e
+
+
+def run_bjorken(lib, CFLtau, integrator="ssprk2", tau0=0.4, tf=4.0, T0=0.5, alpha0=-4.2):
+    g = F.make_grid2d(24, 24, xmax=6.0, ymax=6.0)
+    m = F.build_model_2d(eos=F.LatticeHRGEOS())
+    s = F.Solver(g, m, lib=lib)
+    assert s.init_from_prims(tau0, **ic.uniform(g, T0, alpha0)) == 0
+    U0 = s.get_state()
+    r = s.run_sim_2d(tau0=tau0, taufinal=tf, CFL=0.2, CFLtau=CFLtau, integrator=integrator)
+    assert r["ok"]
+    U = s.get_state().reshape(g.Nxtot, g.Nytot, -1)
+    sx, sy = g.interior()
+    Ui = U[sx, sy]
+    D0 = U0.reshape(g.Nxtot, g.Nytot, -1)[g.nghost, g.nghost, 0]
+    w = s.get_work(("yT", "ux", "uy", "n"))
+    ic_ = g.lin(g.nghost + g.Nx // 2 - 1, g.nghost + g.Ny // 2 - 1)
+    return dict(tau=r["tau"], nprimfail=r["nprimfail"],
+                maxDerr=np.max(np.abs(Ui[..., 0] - D0)) / abs(D0),
+                maxU=max(np.max(np.abs(w["ux"].reshape(g.Nxtot, g.Nytot)[sx, sy])),
+                         np.max(np.abs(w["uy"].reshape(g.Nxtot, g.Nytot)[sx, sy]))),
+                Espread=(Ui[..., 3].max() - Ui[..., 3].min()) / abs(Ui[..., 3].max()),
+                E=U.reshape(-1, U.shape[-1])[ic_, 3], T=np.exp(w["yT"][ic_]), n=w["n"][ic_])
+
+
+def test_bjorken_gate_g0(oracle):
+    ou = OracleUnit(oracle)
+    eos = F.LatticeHRGEOS()
+    r = run_bjorken(oracle, 0.02)
+    assert r["nprimfail"] == 0
+    assert r["maxDerr"] < 1e-13 and r["maxU"] == 0.0 and r["Espread"] < 1e-13
+    r = run_bjorken(oracle, 0.005)
+    T, n, e = _bjorken_exact(ou, eos, 0.5, -4.2 * 0.5, 0.4, r["tau"])
+    assert abs(r["E"] - e) / e < 5e-4 and abs(r["T"] - T) / T < 5e-4 and abs(r["n"] - n) / n < 5e-4
+    for integ, expect in (("ssprk2", 2.0), ("ssprk3", 3.0)):
+        errs = []
+        for c in (0.02, 0.01, 0.005):
+            rr = run_bjorken(oracle, c, integ)
+            errs.append(abs(rr["E"] - _bjorken_exact(ou, eos, 0.5, -2.1, 0.4, rr["tau"])[2]))
+        p2 = np.log2(errs[1] / errs[2])
+        assert p2 > expect - 0.6, (integ, errs)
