@@ -1,0 +1,167 @@
+"""Benchmark of the 2+1D finite-volume hot path on one MI355X.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c1|c2|c3] [--no-cpu]
+
+A step is one accepted time step of run_sim_2d! (main2D.jl:436-513): every RK
+stage's primitive pass, fused face/update kernel and ghost fill, plus (with
+dissipation) the primitive refresh and the relaxation. Inputs are resident in
+HBM when the timed region starts. `value` is cell-updates/s = steps * Nx*Ny /
+wall time (bench/bench2d.jl:52). Prints ONE JSON line.
+
+roofline: the dominant kernel's algorithmic HBM bytes per launch (the per-cell
+model of the workload's bytes/cell-update, restricted to that kernel) over its
+average duration from HIP events on the library's stream.
+cpu_baseline: the project's FP64 C++ restatement of the reference solver (the
+parity oracle), OpenMP over x-columns on the host cores, on a bounded sample of
+the same workload.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "fivohydro.jl_amd"))
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+# Workloads: (description, builder). Algorithmic HBM bytes per cell-update and per
+# dominant-kernel launch follow the per-array model: every array of a pass is read
+# or written once per interior cell.
+CONFIGS = {
+    "c1": dict(
+        workload="gubser-conformal-128x128",
+        desc="2+1D ideal conformal (e=3P) Gubser flow, q=1/fm, tau0=1 fm/c, x,y in [-10,10] fm, "
+             "128x128, SSPRK3, CFLtau=0.4, transverse CFL 0.2, FP64, n=0, dissipation off",
+        N=128, L=10.0, integrator="ssprk3", tau0=1.0, cfl=0.2, cfltau=0.4,
+        nv=4, stages=3, relax=0,
+    ),
+}
+
+
+def algorithmic_bytes(cfg):
+    """Bytes per cell-update: RK state nv*(2 + 3(s-1)) doubles + 8 doubles of Newton
+    warm start per primitive pass (s + m passes) + relaxation (nv + (nv-4) + 12) * m."""
+    nv, s, m = cfg["nv"], cfg["stages"], cfg["relax"]
+    per = nv * (2 + 3 * (s - 1)) + 8 * (s + m) + m * (nv + (nv - 4) + 12)
+    return 8 * per
+
+
+def faces_bytes_per_launch(cfg, ncell):
+    """The fused face/update kernel carries the RK-state traffic of the stage it
+    runs: 2 nv doubles (stage 1) or 3 nv (weighted stages); averaged per launch."""
+    nv, s = cfg["nv"], cfg["stages"]
+    return 8 * nv * (2 + 3 * (s - 1)) / s * ncell
+
+
+def make_problem(cfg, lib):
+    import fivo2d as F
+    from fivo2d import ic
+    N, L = cfg["N"], cfg["L"]
+    g = F.make_grid2d(N, N, xmax=L, ymax=L)
+    if cfg["workload"].startswith("gubser"):
+        m = F.build_model_2d(eos=F.ConformalHQEOS(g_hq=0.0), with_charge=False)
+        prims, _ = ic.gubser_ic(g, cfg["tau0"], 1.0, 0.6, -20.0)
+    s = F.Solver(g, m, lib=lib)
+    s.init_from_prims(cfg["tau0"], **prims)
+    return g, m, s
+
+
+def run_steps(s, tau, nsteps, cfg, reset):
+    r = s.run_sim_2d(tau0=tau, taufinal=1e9, CFL=cfg["cfl"], CFLtau=cfg["cfltau"],
+                     integrator=cfg["integrator"], reset_history=reset, max_steps=nsteps)
+    assert r["ok"] and r["nsteps"] == nsteps, r
+    return r
+
+
+def cpu_baseline(cfg, budget_s=12.0):
+    import fivo2d as F
+    orc = F.Lib(os.path.join(REPO, "oracle", "liboracle2d.so"), "orc_")
+    orc.dll.orc_set_threads.restype = C.c_int
+    orc.dll.orc_set_threads.argtypes = [C.c_int]
+    ncores = os.cpu_count() or 1
+    threads = orc.dll.orc_set_threads(ncores)
+    g, m, s = make_problem(cfg, orc)
+    r = run_steps(s, cfg["tau0"], 2, cfg, True)           # warm-up
+    tau = r["tau"]
+    t0 = time.perf_counter()
+    r = run_steps(s, tau, 3, cfg, False)
+    dt = time.perf_counter() - t0
+    n = int(max(3, min(5000, budget_s / max(dt / 3, 1e-9))))
+    t0 = time.perf_counter()
+    r = run_steps(s, r["tau"], n, cfg, False)
+    wall = time.perf_counter() - t0
+    return {"value": n * g.Nx * g.Ny / wall, "unit": "cell-updates/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{n} steps of {cfg['workload']} ({g.Nx}x{g.Ny}) after a 2-step warm-up, "
+                      f"{wall:.1f} s wall, OpenMP over x-columns"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--config", default="c1", choices=sorted(CONFIGS))
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
+              f"(events shard across GPUs by design only, see DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import fivo2d as F
+    cfg = CONFIGS[a.config]
+    lib = F.load_hip()
+    g, m, s = make_problem(cfg, lib)
+    ncell = g.Nx * g.Ny
+    r = run_steps(s, cfg["tau0"], max(a.warmup, 1), cfg, True)
+    tau = r["tau"]
+    lib.enable_timing(s.ctx, 1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = run_steps(s, tau, a.steps, cfg, False)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    value = a.steps * ncell / wall
+    # dominant kernel from the event timings of the timed region
+    tms = {}
+    for cat, name in ((0, "k_prim"), (1, "k_faces"), (2, "k_bc"), (3, "relax")):
+        tot = C.c_double(0)
+        cnt = lib.kernel_time(s.ctx, cat, C.byref(tot))
+        if cnt > 0:
+            tms[name] = (tot.value, cnt)
+    dom = max(tms, key=lambda k: tms[k][0])
+    avg_ms = tms[dom][0] / tms[dom][1]
+    if dom == "k_faces":
+        alg = faces_bytes_per_launch(cfg, ncell)
+    elif dom == "k_prim":
+        alg = 8 * 8 * g.Ntot          # Newton warm starts, 8 doubles per cell per pass
+    else:
+        alg = 8 * cfg["nv"] * 2 * ncell
+    achieved = alg / (avg_ms * 1e-3) / 1e9
+    out = {
+        "metric": "cell-updates/s, plain vs batched+learned, 512² viscous+charm, ΔT≤1e-9",
+        "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": wall / a.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": cfg["workload"], "grid": f"{g.Nx}x{g.Ny}", "integrator": cfg["integrator"],
+                   "path": "plain", "description": cfg["desc"],
+                   "alg_bytes_per_cell_update": algorithmic_bytes(cfg),
+                   "kernel_ms": {k: round(v[0] / v[1], 5) for k, v in tms.items()},
+                   "kernel_launches": {k: v[1] for k, v in tms.items()}},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "step_achieved_GBs": algorithmic_bytes(cfg) * value / 1e9},
+    }
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(cfg)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

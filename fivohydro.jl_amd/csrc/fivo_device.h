@@ -91,7 +91,7 @@ __device__ __forceinline__ double k1x_big_Q3(double a) { return FD_H(a, 1.0, 1.4
 
 // e^x K0 and e^x K1 together: the large-x branch shares 1/x and sqrt(x),
 // the small-x branch shares log(x) and exp(x).
-__device__ __forceinline__ void besselk01x(double x, double* k0, double* k1) {
+__device__ __noinline__ void besselk01x(double x, double* k0, double* k1) {
     if (x <= 1.0) {
         double lx = log(x), ex = exp(x), z = x * x, a = z / 4;
         double s = fma(k0x_small_P1(a), 1.0 / k0x_small_Q1(a), 1.137250900268554688);
@@ -125,6 +125,14 @@ __device__ __forceinline__ double safe_besselkx(int nu, double x) {
     if (nu == 0) return km;
     for (int n = 1; n < nu; ++n) { double kp = km + (2.0 * n / x) * k; km = k; k = kp; }
     return k;
+}
+// safe_besselkx(1, x) and safe_besselkx(2, x) from one K0/K1 evaluation
+__device__ __forceinline__ void besselk12x(double x, double* k1, double* k2) {
+    if (!(isf(x) && x > 0.0)) { *k1 = 0.0; *k2 = 0.0; return; }
+    if (x >= ASYM_X) { *k1 = besselkx_asym(1, x); *k2 = besselkx_asym(2, x); return; }
+    double k0;
+    besselk01x(x, &k0, k1);
+    *k2 = k0 + (2.0 * 1 / x) * *k1;
 }
 // safe_besselk2x_2d (src2d/bessel2d.jl:111-115)
 __device__ __forceinline__ double besselk2x_fast(double x) {
@@ -187,16 +195,19 @@ __device__ __forceinline__ void light_eval(double T, const EosP& q, double* P, d
 // T-only part of the EOS at T (>= T_MIN): what the reference memoises per T
 struct TPart { double T, Pl, el, K2, K1, x; };
 
+// EOS kinds as template parameter EK: 0 = conformal without the charm sector
+// (g_hq = 0), 2 = conformal with it, 1 = lattice-HRG.
 template <int EK>
 __device__ __forceinline__ TPart eos_tpart(double T, const EosP& q) {
+    constexpr bool CH = EK != 0;
     TPart t;
     t.T = T;
-    if (EK == 0) {
+    if (EK != 1) {
         t.Pl = q.aSB * pow4(T) * q.fmGeV3;
         t.el = 3.0 * t.Pl;
         t.x = q.m / T;
         t.K2 = 0.0; t.K1 = 0.0;
-        if (q.conf_charm) { t.K2 = safe_besselkx(2, t.x); t.K1 = safe_besselkx(1, t.x); }
+        if (CH) besselk12x(t.x, &t.K1, &t.K2);
     } else {
         double P, dP, d2P;
         light_eval<1>(T, q, &P, &dP, &d2P);
@@ -213,9 +224,10 @@ __device__ __forceinline__ TPart eos_tpart(double T, const EosP& q) {
 template <int EK>
 __device__ __forceinline__ void eos_assemble(const TPart& t, double mu, const EosP& q, double* P, double* n,
                                              double* e) {
+    constexpr bool CH = EK != 0;
     const double T = t.T;
-    if (EK == 0) {
-        if (!q.conf_charm || !isf(t.x) || t.x <= 0.0) { *P = t.Pl; *n = 0.0; *e = t.el; return; }
+    if (EK != 1) {
+        if (!CH || !isf(t.x) || t.x <= 0.0) { *P = t.Pl; *n = 0.0; *e = t.el; return; }
         double Efac = exp((mu - q.m) / T);
         double nh = q.Aconf * T * Efac * t.K2;
         double Ph = q.Aconf * (T * T) * Efac * t.K2;
@@ -239,10 +251,10 @@ __device__ __forceinline__ void eos_Pne_2d(double T, double mu, const EosP& q, d
 template <int EK>
 __device__ __forceinline__ void eos_Pe_2d(double T, double mu, const EosP& q, double* P, double* e) {
     T = jmax(T, T_MIN);
-    if (EK == 0) {
+    if (EK != 1) {
         double n;
-        TPart t = eos_tpart<0>(T, q);
-        eos_assemble<0>(t, mu, q, P, &n, e);
+        TPart t = eos_tpart<EK>(T, q);
+        eos_assemble<EK>(t, mu, q, P, &n, e);
     } else {
         double Pl, dP, d2P;
         light_eval<1>(T, q, &Pl, &dP, &d2P);
@@ -253,7 +265,7 @@ __device__ __forceinline__ void eos_Pe_2d(double T, double mu, const EosP& q, do
 // eos_cs2 (src/eos.jl:49, 471-476)
 template <int EK>
 __device__ __forceinline__ double eos_cs2(double T, const EosP& q) {
-    if (EK == 0) return 1.0 / 3.0;
+    if (EK != 1) return 1.0 / 3.0;
     T = jmax(T, T_MIN);
     double P, dP, d2P;
     light_eval<2>(T, q, &P, &dP, &d2P);

@@ -137,9 +137,9 @@ __device__ __forceinline__ bool p2c(double T, double mu, double ux, double uy, c
     const double Tc = jmax(T, T_MIN);
     const double ut = sqrt(1 + ux * ux + uy * uy);
     double P, n, e, cs2;
-    if (EK == 0) {
-        TPart t = eos_tpart<0>(Tc, q);
-        eos_assemble<0>(t, mu, q, &P, &n, &e);
+    if (EK != 1) {
+        TPart t = eos_tpart<EK>(Tc, q);
+        eos_assemble<EK>(t, mu, q, &P, &n, &e);
         cs2 = 1.0 / 3.0;
     } else {
         double Pl, dP, d2P;
@@ -422,22 +422,22 @@ __global__ __launch_bounds__(TX* TY) void k_faces(FaceK K) {
         const int nxt = min(TX, p.ng + p.Nx - x0), nyt = min(TY, p.ng + p.Ny - y0);
         const uint8_t* force = K.f.force;
         for (int f = tid; f < NF; f += NT) {
+            // one call site for both directions keeps a single copy of the face code
+            int i = 0, dir = 0, k0 = 0, st = 0;
+            bool valid;
             if (f < NFX) {
                 const int fx = f / TY, fy = f - fx * TY;
-                if (fx <= nxt && fy < nyt) {
-                    const int ixl = x0 + fx - 1, iy = y0 + fy, i = ixl * p.Nytot + iy;
-                    const bool forced = force && (force[i] || force[i + p.Nytot]);
-                    double a = face<EK, NCH, true>(K, i, 0, ixl, forced, sF + f, NF);
-                    amax = a > amax ? a : amax;
-                }
+                valid = fx <= nxt && fy < nyt;
+                k0 = x0 + fx - 1; i = k0 * p.Nytot + y0 + fy; st = p.Nytot;
             } else {
                 const int g = f - NFX, gx = g / (TY + 1), gy = g - gx * (TY + 1);
-                if (gx < nxt && gy <= nyt) {
-                    const int ix = x0 + gx, iyl = y0 + gy - 1, i = ix * p.Nytot + iyl;
-                    const bool forced = force && (force[i] || force[i + 1]);
-                    double a = face<EK, NCH, true>(K, i, 1, iyl, forced, sF + f, NF);
-                    amax = a > amax ? a : amax;
-                }
+                valid = gx < nxt && gy <= nyt;
+                k0 = y0 + gy - 1; i = (x0 + gx) * p.Nytot + k0; dir = 1; st = 1;
+            }
+            if (valid) {
+                const bool forced = force && (force[i] || force[i + st]);
+                double a = face<EK, NCH, true>(K, i, dir, k0, forced, sF + f, NF);
+                amax = a > amax ? a : amax;
             }
         }
         __syncthreads();
@@ -680,7 +680,7 @@ __global__ __launch_bounds__(256) void k_project_shear(double* U, Params p, DevW
 #define EOS_DISPATCH(c, BODY)                                                    \
     do {                                                                         \
         if ((c).p.eos.kind == 1) { constexpr int EK = 1; constexpr bool NCH = true; BODY; } \
-        else if ((c).p.eos.has_charge) { constexpr int EK = 0; constexpr bool NCH = true; BODY; } \
+        else if ((c).p.eos.has_charge) { constexpr int EK = 2; constexpr bool NCH = true; BODY; } \
         else { constexpr int EK = 0; constexpr bool NCH = false; BODY; }         \
     } while (0)
 
@@ -750,10 +750,8 @@ hipError_t launch_admissible_pending(Ctx& c, const double* U, double tau) {
 hipError_t launch_init_prims(Ctx& c, double* U, double tau0, const double* const* prims, unsigned long long*) {
     InitK in;
     for (int k = 0; k < 10; ++k) in.a[k] = prims[k];
-    if (c.p.eos.kind == 1)
-        hipLaunchKernelGGL((k_init<1>), dim3(nblk(c.p.Ntot, 256)), dim3(256), 0, c.stream, U, c.p, tau0, in, c.sc);
-    else
-        hipLaunchKernelGGL((k_init<0>), dim3(nblk(c.p.Ntot, 256)), dim3(256), 0, c.stream, U, c.p, tau0, in, c.sc);
+    EOS_DISPATCH(c, hipLaunchKernelGGL((k_init<EK>), dim3(nblk(c.p.Ntot, 256)), dim3(256), 0, c.stream, U, c.p,
+                                       tau0, in, c.sc));
     return hipGetLastError();
 }
 
