@@ -44,6 +44,7 @@ fd::EosP make_eos(const fivo_eos_desc& e) {
     q.kind = e.kind;
     q.hq_fm = e.hq_times_fmGeV3 != 0;
     q.m = e.m_hq;
+    q.ghq = e.g_hq;
     q.fmGeV3 = kFMGEV3;
     q.aSB = (kPI2 / 90) * e.g_eff;
     q.Aconf = e.g_hq * (e.m_hq * e.m_hq) / (2 * kPI2);

@@ -150,6 +150,7 @@ struct EosP {
     int hq_fm;
     int conf_charm;      // conformal with g_hq > 0: charm sector evaluated
     double m;            // hq_mass
+    double ghq;          // g_hq (charm degeneracy)
     double aSB;          // (pi^2/90) g_eff
     double Aconf;        // g m^2 / (2 pi^2)
     double Alhrg;        // canon g m^2 / (2 pi^2)
