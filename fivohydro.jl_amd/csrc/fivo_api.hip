@@ -105,9 +105,9 @@ void resolve_timing(Ctx& c) {
 }
 
 int reset_scalars(Ctx& c) {
-    // amax, nbad, nprimfail, nvacuum, npending -> 0 (newton_iters kept)
+    // amax, nbad, nprimfail, nvacuum, npending, ncold -> 0 (newton_iters kept)
     HIPCHK(&c, hipMemsetAsync(c.sc, 0, 4 * sizeof(unsigned long long), c.stream));
-    HIPCHK(&c, hipMemsetAsync(&c.sc->npending, 0, sizeof(unsigned long long), c.stream));
+    HIPCHK(&c, hipMemsetAsync(&c.sc->npending, 0, 2 * sizeof(unsigned long long), c.stream));
     return FIVO_OK;
 }
 int read_scalars(Ctx& c) {
@@ -443,7 +443,7 @@ int fivo_create(const fivo_grid_desc* g, const fivo_model_desc* m, fivo_ctx** ou
     }
     const size_t P = p.pitch;
     const size_t ndbl = 4 * nv * P + nv * P + nv * P + 24 * P + 3 * (size_t)c->npartials + 64;
-    const size_t nbyte = ndbl * sizeof(double) + 4 * P + 256;
+    const size_t nbyte = ndbl * sizeof(double) + 4 * P + 4 * P + 256;
     c->pool_bytes = nbyte;
     if ((e = hipMalloc(&c->pool, nbyte)) != hipSuccess) {
         (void)hipStreamDestroy(c->stream); delete c;
@@ -465,6 +465,7 @@ int fivo_create(const fivo_grid_desc* g, const fivo_model_desc* m, fivo_ctx** ou
     c->w.bad = b; b += P;
     c->w.force = b; b += P;
     c->w.ftmp = b; b += P;
+    c->coldlist = reinterpret_cast<int*>(b); b += 4 * P;
     if ((e = hipMalloc(&c->sc, sizeof(DevScalars))) != hipSuccess ||
         (e = hipHostMalloc(&c->sc_host, sizeof(DevScalars), hipHostMallocDefault)) != hipSuccess ||
         (e = hipEventCreate(&c->ev[0])) != hipSuccess || (e = hipEventCreate(&c->ev[1])) != hipSuccess) {

@@ -49,7 +49,7 @@ struct DevScalars {
     unsigned long long newton_iters;// cumulative
     unsigned long long shear_max_bits;
     unsigned long long npending;    // near-bound cells awaiting the inversion test
-    unsigned long long pad;
+    unsigned long long ncold;       // cells of the current primitive pass needing a cold seed
 };
 
 // Per-cell work arrays on the device (Work2D, src2d/work2d.jl:15-100)
@@ -77,6 +77,7 @@ struct Ctx {
     double* dU = nullptr;           // RHS output for fivo_rhs
     double* snap = nullptr;         // pre-relaxation snapshot (dissipation2d.jl:299-303)
     double* partials = nullptr;     // reduction partials
+    int* coldlist = nullptr;        // cells of a primitive pass solved from the cold seed
     int npartials = 0;
     DevWork w;
     DevScalars* sc = nullptr;       // device

@@ -434,7 +434,9 @@ __device__ __noinline__ bool solve_phi_bisect(double D, double ux, double uy, do
 }
 
 // _cons_to_prim_attempt_2d! (src2d/primrec2d.jl:511-742). NCH: eos_has_charge_2d.
-template <int EK, bool NCH>
+// SEED = false instantiates the warm-started solve only: the caller guarantees a
+// finite warm start (or a vacuum state), so the cold-seed code is not compiled in.
+template <int EK, bool NCH, bool SEED = true>
 __device__ __forceinline__ PrimRes cons_to_prim_attempt(double D, double Sx, double Sy, double E, const Diss& d,
                                                      double yT0, double phi0, double ux0, double uy0,
                                                      const EosP& q) {
@@ -456,7 +458,7 @@ __device__ __forceinline__ PrimRes cons_to_prim_attempt(double D, double Sx, dou
         x[0] = jclamp(yT0, yT_lo, yT_hi);
         x[2] = jclamp(ux0, -U_CAP, U_CAP);
         x[3] = jclamp(uy0, -U_CAP, U_CAP);
-    } else {
+    } else if (SEED) {
         double sy, sx, syy;
         seed_from_conserved<EK>(E, Sx, Sy, x[1], q, &sy, &sx, &syy);
         x[0] = jclamp(sy, yT_lo, yT_hi); x[2] = sx; x[3] = syy;
@@ -592,7 +594,7 @@ __device__ __forceinline__ PrimRes cons_to_prim(double D, double Sx, double Sy, 
     const double nan = __longlong_as_double(0x7ff8000000000000LL);
 #pragma unroll 1
     for (int pass = 0; pass < 2; ++pass) {
-        r = cons_to_prim_attempt<EK, NCH>(D, Sx, Sy, E, d, yT0, phi0, ux0, uy0, q);
+        r = cons_to_prim_attempt<EK, NCH, true>(D, Sx, Sy, E, d, yT0, phi0, ux0, uy0, q);
         its += r.iters;
         if (r.ok || r.reason == PRR_VACUUM || !warm) break;
         yT0 = nan; phi0 = nan; ux0 = nan; uy0 = nan;

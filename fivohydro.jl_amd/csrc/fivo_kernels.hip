@@ -12,6 +12,7 @@
 //   + small kernels: standalone sanitiser / admissibility, IC forward map,
 //     history snapshot, diagnostics reductions, shear projection, layout copies.
 // =============================================================================
+#include <algorithm>
 #include "fivo_ctx.h"
 
 namespace fivo {
@@ -79,51 +80,114 @@ __global__ __launch_bounds__(256) void k_bc(double* __restrict__ U, Params p, in
 }
 
 // ---- primitive pass (src2d/rhs2d.jl:67-160) ---------------------------------------
-template <int EK, bool NCH>
-__global__ __launch_bounds__(256) void k_prim(const double* __restrict__ U, Params p, double tau, DevWork w,
-                                              DevScalars* sc) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    unsigned long long its = 0, nfail = 0, nvac = 0;
-    if (i < p.Ntot) {
-        const double D = safe_div(U[i], tau), Sx = U[P_ + i], Sy = U[2 * P_ + i], E = U[3 * P_ + i];
-        const Diss d = load_diss(U, p, i);
-        PrimRes r = cons_to_prim<EK, NCH>(D, Sx, Sy, E, d, w.x0_yT[i], w.x0_phi[i], w.x0_ux[i], w.x0_uy[i],
-                                          p.eos);
-        its = r.iters;
-        double T = r.T, mu = r.mu, ux = r.ux, uy = r.uy, n = r.n, e = r.e, P = r.P;
-        if (!r.ok) {
-            if (r.reason == PRR_VACUUM) nvac = 1; else nfail = 1;
-            T = T_MIN; mu = 0.0; ux = 0.0; uy = 0.0;
-            eos_Pne_2d<EK>(T, mu, p.eos, &P, &n, &e);
-            e = jmax(e, 0.0); n = 0.0;
+// Two kernels. k_prim runs every cell whose Newton starts warm (the production
+// path) or is vacuum; a cell with no warm start, or whose warm solve failed (the
+// fresh-seed retry of cons_to_prim_2d!, src2d/primrec2d.jl:465-486), is appended
+// to a worklist that k_prim_cold solves from the cold seed. The cold seed costs
+// ~180 EOS calls; kept out of k_prim it no longer stalls the 63 warm lanes of its
+// wavefront, and k_prim carries no seed code (fewer registers).
+__device__ __forceinline__ void prim_finalize(const Params& p, const DevWork& w, int i, const PrimRes& r0,
+                                              const Diss& d, unsigned long long& nfail, unsigned long long& nvac) {
+    double T = r0.T, mu = r0.mu, ux = r0.ux, uy = r0.uy, n = r0.n, e = r0.e, P = r0.P;
+    if (!r0.ok) {
+        if (r0.reason == PRR_VACUUM) nvac += 1; else nfail += 1;
+        T = T_MIN; mu = 0.0; ux = 0.0; uy = 0.0;
+        // eos_Pne_2d(T_MIN, 0): the light sector at T_MIN, no charm density
+        if (p.eos.kind == 1) {
+            double Pl, dP, d2P;
+            light_eval<1>(T_MIN, p.eos, &Pl, &dP, &d2P);
+            P = Pl + 0.0; e = jmax((-Pl + T_MIN * dP) + 0.0, 0.0);
+        } else {
+            // conformal: the charm sector carries exp(-m/T_MIN) = 0
+            const double Pl = p.eos.aSB * pow4(T_MIN) * p.eos.fmGeV3, el = 3.0 * Pl;
+            double Ph = 0.0, eh = 0.0;
+            if (p.eos.conf_charm) {
+                const double x = p.eos.m / T_MIN;
+                const double K2 = besselkx_asym(2, x), K1 = besselkx_asym(1, x);
+                const double Ef = exp((0.0 - p.eos.m) / T_MIN);
+                Ph = p.eos.Aconf * (T_MIN * T_MIN) * Ef * K2;
+                eh = p.eos.Aconf * (T_MIN * T_MIN) * Ef * (3.0 * K2 + x * K1);
+                if (p.eos.hq_fm) { Ph *= p.eos.fmGeV3; eh *= p.eos.fmGeV3; }
+            }
+            P = Pl + Ph; e = jmax(el + eh, 0.0);
         }
-        const double Tm = jmax(T, T_MIN);
-        const double yT = log(Tm);
-        const double phi = jclamp((mu - p.eos.m) / Tm, -PHI_CAP, PHI_CAP);
-        w.yT[i] = yT; w.phi[i] = phi; w.mu[i] = mu; w.alpha[i] = mu / Tm;
-        w.ux[i] = ux; w.uy[i] = uy; w.P[i] = P; w.n[i] = n; w.e[i] = e;
-        w.ok[i] = r.ok ? 1 : 0;
-        const double ut = sqrt(1 + ux * ux + uy * uy), iut = safe_inv(ut);
-        w.vx[i] = ux * iut; w.vy[i] = uy * iut;
-        const double nan = __longlong_as_double(0x7ff8000000000000LL);
-        w.x0_yT[i] = r.ok ? yT : nan;
-        w.x0_phi[i] = r.ok ? phi : nan;
-        w.x0_ux[i] = r.ok ? ux : nan;
-        w.x0_uy[i] = r.ok ? uy : nan;
-        if (p.hasShear) {
-            double tt, tx, ty;
-            shear_tau_row(ux, uy, ut, d.pixx, d.pixy, d.piyy, &tt, &tx, &ty);
-            double R = -tt + d.pixx + d.piyy + d.pieta;
-            double scale = jmax(jmax(jmax(jmax(fabs(tt), fabs(d.pixx)), fabs(d.piyy)), fabs(d.pieta)), fabs(d.pixy));
-            w.shear_res[i] = scale > TINY ? fabs(R) / scale : 0.0;
-        }
+        n = 0.0;
     }
+    const double Tm = jmax(T, T_MIN);
+    const double yT = log(Tm);
+    const double phi = jclamp((mu - p.eos.m) / Tm, -PHI_CAP, PHI_CAP);
+    w.yT[i] = yT; w.phi[i] = phi; w.mu[i] = mu; w.alpha[i] = mu / Tm;
+    w.ux[i] = ux; w.uy[i] = uy; w.P[i] = P; w.n[i] = n; w.e[i] = e;
+    w.ok[i] = r0.ok ? 1 : 0;
+    const double ut = sqrt(1 + ux * ux + uy * uy), iut = safe_inv(ut);
+    w.vx[i] = ux * iut; w.vy[i] = uy * iut;
+    const double nan = __longlong_as_double(0x7ff8000000000000LL);
+    w.x0_yT[i] = r0.ok ? yT : nan;
+    w.x0_phi[i] = r0.ok ? phi : nan;
+    w.x0_ux[i] = r0.ok ? ux : nan;
+    w.x0_uy[i] = r0.ok ? uy : nan;
+    if (p.hasShear) {
+        double tt, tx, ty;
+        shear_tau_row(ux, uy, ut, d.pixx, d.pixy, d.piyy, &tt, &tx, &ty);
+        double R = -tt + d.pixx + d.piyy + d.pieta;
+        double scale = jmax(jmax(jmax(jmax(fabs(tt), fabs(d.pixx)), fabs(d.piyy)), fabs(d.pieta)), fabs(d.pixy));
+        w.shear_res[i] = scale > TINY ? fabs(R) / scale : 0.0;
+    }
+}
+
+__device__ __forceinline__ void prim_counts(DevScalars* sc, unsigned long long its, unsigned long long nfail,
+                                            unsigned long long nvac) {
     its = wave_sum(its); nfail = wave_sum(nfail); nvac = wave_sum(nvac);
     if ((threadIdx.x & 63) == 0) {
         if (its) atomicAdd(&sc->newton_iters, its);
         if (nfail) atomicAdd(&sc->nprimfail, nfail);
         if (nvac) atomicAdd(&sc->nvacuum, nvac);
     }
+}
+
+template <int EK, bool NCH>
+__global__ __launch_bounds__(256) void k_prim(const double* __restrict__ U, Params p, double tau, DevWork w,
+                                              DevScalars* sc, int* coldlist) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long its = 0, nfail = 0, nvac = 0;
+    if (i < p.Ntot) {
+        const double D = safe_div(U[i], tau), Sx = U[P_ + i], Sy = U[2 * P_ + i], E = U[3 * P_ + i];
+        const Diss d = load_diss(U, p, i);
+        const double yT0 = w.x0_yT[i], ux0 = w.x0_ux[i], uy0 = w.x0_uy[i];
+        const bool vac = !isf(E) || E <= E_VAC || !isf(D) || D < 0.0;
+        const bool warm = isf(yT0) && isf(ux0) && isf(uy0);
+        bool done = true;
+        if (vac || warm) {
+            PrimRes r = cons_to_prim_attempt<EK, NCH, false>(D, Sx, Sy, E, d, yT0, w.x0_phi[i], ux0, uy0, p.eos);
+            its = r.iters;
+            if (r.ok || r.reason == PRR_VACUUM) prim_finalize(p, w, i, r, d, nfail, nvac);
+            else done = false;                       // retry from the fresh seed
+        } else {
+            done = false;
+        }
+        if (!done) {
+            const unsigned long long k = atomicAdd(&sc->ncold, 1ull);
+            coldlist[k] = i;
+        }
+    }
+    prim_counts(sc, its, nfail, nvac);
+}
+
+template <int EK, bool NCH>
+__global__ __launch_bounds__(64) void k_prim_cold(const double* __restrict__ U, Params p, double tau, DevWork w,
+                                                  DevScalars* sc, const int* coldlist) {
+    const int n = (int)sc->ncold;
+    unsigned long long its = 0, nfail = 0, nvac = 0;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        const int i = coldlist[t];
+        const double D = safe_div(U[i], tau), Sx = U[P_ + i], Sy = U[2 * P_ + i], E = U[3 * P_ + i];
+        const Diss d = load_diss(U, p, i);
+        const double nan = __longlong_as_double(0x7ff8000000000000LL);
+        PrimRes r = cons_to_prim_attempt<EK, NCH, true>(D, Sx, Sy, E, d, nan, nan, nan, nan, p.eos);
+        its += r.iters;
+        prim_finalize(p, w, i, r, d, nfail, nvac);
+    }
+    prim_counts(sc, its, nfail, nvac);
 }
 
 // ---- face physics -------------------------------------------------------------------
@@ -293,8 +357,6 @@ __device__ __forceinline__ double face(const FaceK& K, int i, int dir, int k0, b
             row(p.iNux, zL.nux, zR.nux); row(p.iNuy, zL.nuy, zR.nuy); row(p.iPi, zL.Pi, zR.Pi);
             row(p.iPixx, zL.pixx, zR.pixx); row(p.iPixy, zL.pixy, zR.pixy);
             row(p.iPiyy, zL.piyy, zR.piyy); row(p.iPieta, zL.pieta, zR.pieta);
-            if (p.iPQ0 >= 0)
-                for (int k = 0; k < 5; ++k) out[(p.iPQ0 + k) * ostride] = 0.0;
         }
     }
     const double aa = jmax(fabs(a_l), fabs(a_r));
@@ -461,7 +523,9 @@ __global__ __launch_bounds__(TX* TY) void k_faces(FaceK K) {
                 dU[P_ + i] = du(1) + S1;
                 dU[2 * P_ + i] = du(2) + S2;
                 dU[3 * P_ + i] = du(3) + S3;
-                for (int a = 4; a < p.nv; ++a) dU[a * P_ + i] = du(a) + 0.0;
+                const int nvf = p.iPQ0 >= 0 ? p.iPQ0 : p.nv;
+                for (int a = 4; a < nvf; ++a) dU[a * P_ + i] = du(a) + 0.0;
+                for (int a = nvf; a < p.nv; ++a) dU[a * P_ + i] = 0.0;
             } else {
                 const double D = K.f.D, wo = K.f.wold, wn = K.f.wnew;
                 const int ct = K.f.combine;
@@ -485,9 +549,10 @@ __global__ __launch_bounds__(TX* TY) void k_faces(FaceK K) {
                 s.piyy = p.hasShear ? comb(p.iPiyy, b.piyy, du(p.iPiyy) + 0.0) : 0.0;
                 s.pieta = p.hasShear ? comb(p.iPieta, b.pieta, du(p.iPieta) + 0.0) : 0.0;
                 if (p.iPQ0 >= 0) {
+                    // charm second-moment rows: never in the face states, zero flux
                     for (int k = 0; k < 5; ++k) {
                         const int a = p.iPQ0 + k;
-                        K.f.Uout[a * P_ + i] = comb(a, Ub[a * P_ + i], du(a) + 0.0);
+                        K.f.Uout[a * P_ + i] = comb(a, Ub[a * P_ + i], 0.0);
                     }
                 }
                 // sanitize_stage_2d! at tau_state (src2d/floors2d.jl:218-227)
@@ -695,8 +760,12 @@ hipError_t launch_bc(Ctx& c, double* U, int bc) {
 
 hipError_t launch_prim(Ctx& c, const double* U, double tau) {
     const Params& p = c.p;
+    // the worklist counter is cleared with the other per-pass scalars
     EOS_DISPATCH(c, hipLaunchKernelGGL((k_prim<EK, NCH>), dim3(nblk(p.Ntot, 256)), dim3(256), 0, c.stream, U, p,
-                                       tau, c.w, c.sc));
+                                       tau, c.w, c.sc, c.coldlist));
+    const int nb = std::min(nblk(p.Ntot, 64), 4096);
+    EOS_DISPATCH(c, hipLaunchKernelGGL((k_prim_cold<EK, NCH>), dim3(nb), dim3(64), 0, c.stream, U, p, tau, c.w,
+                                       c.sc, c.coldlist));
     return hipGetLastError();
 }
 
@@ -712,7 +781,15 @@ static hipError_t faces_tx(Ctx& c, const FaceLaunch& f) {
     const int nextra = 2 * ng * (p.Nxtot - 1) + 2 * p.Ny * (ng - 1) + 2 * ng * (p.Nytot - 1) + 2 * p.Nx * (ng - 1);
     const int nb = K.nmain + nblk(nextra, TX * TY);
     constexpr int NF = (TX + 1) * TY + TX * (TY + 1);
-    const size_t lds = sizeof(double) * (size_t)p.nv * NF;
+    const int nvf = p.iPQ0 >= 0 ? p.iPQ0 : p.nv;   // rows with a face flux
+    const size_t lds = sizeof(double) * (size_t)nvf * NF;
+    static size_t lds_set = 0;
+    if (lds > 65536 && lds > lds_set) {
+        hipError_t e = hipFuncSetAttribute((const void*)k_faces<EK, NCH, TX, TY>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+        lds_set = lds;
+    }
     hipLaunchKernelGGL((k_faces<EK, NCH, TX, TY>), dim3(nb), dim3(TX * TY), lds, c.stream, K);
     return hipGetLastError();
 }

@@ -102,6 +102,11 @@ int orc_set_work(Ctx* c, const fivo_work_view* v) {
     return FIVO_OK;
 }
 int orc_reset_history(Ctx* c) { reset_history(*c); return FIVO_OK; }
+// diagnostics: enable per-cell Newton iteration recording; copy the histogram out
+void orc_debug_iters(Ctx* c, int enable, int64_t* hist256) {
+    if (enable) c->w.dbg_iters.assign(c->g.Ntot, -1); else c->w.dbg_iters.clear();
+    if (hist256) for (int k = 0; k < 256; ++k) hist256[k] = k < (int)c->w.dbg_hist.size() ? c->w.dbg_hist[k] : 0;
+}
 
 // finalize_ic! (main2D.jl:164-171)
 int orc_finalize_ic(Ctx* c, double tau0, int bc) {

@@ -76,6 +76,7 @@ void primitive_pass(Ctx& c, std::vector<double>& U, double tau) {
             PrimOut r = cons_to_prim(cs, d, tau, q, w.x0_yT[i], w.x0_phi[i], w.x0_ux[i], w.x0_uy[i],
                                      st, &its);
             nit += its;
+            if (!w.dbg_iters.empty()) w.dbg_iters[i] = (st.reason == PRR_VACUUM) ? -1 : its;
             double T = r.T, mu = r.mu, ux = r.ux, uy = r.uy, n = r.n, e = r.e, P = r.P;
             bool ok = r.ok;
             if (!ok) {
@@ -104,6 +105,10 @@ void primitive_pass(Ctx& c, std::vector<double>& U, double tau) {
         }
     }
     w.primfail = nfail; w.vacuum = nvac; w.newton_iters += nit;
+    if (!w.dbg_iters.empty()) {
+        if (w.dbg_hist.size() < 256) w.dbg_hist.assign(256, 0);
+        for (int v : w.dbg_iters) if (v >= 0) w.dbg_hist[v < 255 ? v : 255] += 1;
+    }
 }
 
 // reconstruct_muscl_2d! (src2d/reconstruction2d.jl:27-103); dir 0 = x, 1 = y

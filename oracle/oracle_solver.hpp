@@ -68,6 +68,8 @@ struct Work {
     std::vector<double> S, k, U1, U2, diss_snap, theta;
     std::vector<double> slope_tmp, alphaF[2], gradA[2];
     std::vector<double> shear_res;
+    std::vector<int> dbg_iters;      // Newton iterations of each cell in the last primitive pass
+    std::vector<int64_t> dbg_hist;   // histogram of per-cell iterations over all passes
     int64_t primfail = 0, vacuum = 0, newton_iters = 0;
     double amax = 0.0;
 };
