@@ -443,7 +443,7 @@ int fivo_create(const fivo_grid_desc* g, const fivo_model_desc* m, fivo_ctx** ou
     }
     const size_t P = p.pitch;
     const size_t ndbl = 4 * nv * P + nv * P + nv * P + 24 * P + 3 * (size_t)c->npartials + 64;
-    const size_t nbyte = ndbl * sizeof(double) + 4 * P + 4 * P + 256;
+    const size_t nbyte = ndbl * sizeof(double) + 4 * P + sizeof(PrimJob) * P + 256;
     c->pool_bytes = nbyte;
     if ((e = hipMalloc(&c->pool, nbyte)) != hipSuccess) {
         (void)hipStreamDestroy(c->stream); delete c;
@@ -465,7 +465,7 @@ int fivo_create(const fivo_grid_desc* g, const fivo_model_desc* m, fivo_ctx** ou
     c->w.bad = b; b += P;
     c->w.force = b; b += P;
     c->w.ftmp = b; b += P;
-    c->coldlist = reinterpret_cast<int*>(b); b += 4 * P;
+    c->jobs = reinterpret_cast<PrimJob*>(b); b += sizeof(PrimJob) * P;
     if ((e = hipMalloc(&c->sc, sizeof(DevScalars))) != hipSuccess ||
         (e = hipHostMalloc(&c->sc_host, sizeof(DevScalars), hipHostMallocDefault)) != hipSuccess ||
         (e = hipEventCreate(&c->ev[0])) != hipSuccess || (e = hipEventCreate(&c->ev[1])) != hipSuccess) {

@@ -49,10 +49,15 @@ struct DevScalars {
     unsigned long long newton_iters;// cumulative
     unsigned long long shear_max_bits;
     unsigned long long npending;    // near-bound cells awaiting the inversion test
-    unsigned long long ncold;       // cells of the current primitive pass needing a cold seed
+    unsigned long long ncold;       // cells of the current primitive pass queued for the cooperative solve
 };
 
 // Per-cell work arrays on the device (Work2D, src2d/work2d.jl:15-100)
+// a cell queued by the warm primitive kernel: resume its Newton at iteration
+// `it` from the iterate kept in the warm-start arrays, or (it = 0) solve it
+// from the cold seed (it >= 1000: only the staged fallback is left)
+struct PrimJob { int cell, it; };
+
 struct DevWork {
     double *yT, *phi, *mu, *alpha, *ux, *uy, *P, *n, *e, *vx, *vy;
     uint8_t *ok, *bad, *force, *ftmp;
@@ -77,7 +82,7 @@ struct Ctx {
     double* dU = nullptr;           // RHS output for fivo_rhs
     double* snap = nullptr;         // pre-relaxation snapshot (dissipation2d.jl:299-303)
     double* partials = nullptr;     // reduction partials
-    int* coldlist = nullptr;        // cells of a primitive pass solved from the cold seed
+    PrimJob* jobs = nullptr;        // cells of a primitive pass queued for the cooperative solve
     int npartials = 0;
     DevWork w;
     DevScalars* sc = nullptr;       // device

@@ -90,23 +90,27 @@ __device__ __forceinline__ double k1x_big_Q3(double a) { return FD_H(a, 1.0, 1.4
 #undef FD_H
 
 // e^x K0 and e^x K1 together: the large-x branch shares 1/x and sqrt(x),
-// the small-x branch shares log(x) and exp(x).
-__device__ __noinline__ void besselk01x(double x, double* k0, double* k1) {
+// the small-x branch shares log(x) and exp(x). Out of line (it is large and
+// called from many places); the pair comes back in registers.
+struct K01 { double k0, k1; };
+__device__ __noinline__ K01 besselk01x(double x) {
+    K01 r;
     if (x <= 1.0) {
         double lx = log(x), ex = exp(x), z = x * x, a = z / 4;
         double s = fma(k0x_small_P1(a), 1.0 / k0x_small_Q1(a), 1.137250900268554688);
         double a0 = fma(s, a, 1.0);
-        *k0 = fma(-a0, lx, k0x_small_P2(z)) * ex;
+        r.k0 = fma(-a0, lx, k0x_small_P2(z)) * ex;
         double pq = fma(k1x_small_P1(a), 1.0 / k1x_small_Q1(a), 8.69547128677368164e-2);
         pq = fma(pq * a, a, (a / 2 + 1));
         double a1 = pq * x / 2;
         double pq2 = fma(k1x_small_P2(z) / k1x_small_Q2(z), x, 1.0 / x);
-        *k1 = fma(a1, lx, pq2) * ex;
+        r.k1 = fma(a1, lx, pq2) * ex;
     } else {
         double ix = 1.0 / x, sx = sqrt(x);
-        *k0 = fma(k0x_big_P3(ix), 1.0 / k0x_big_Q3(ix), 1.0) / sx;
-        *k1 = fma(k1x_big_P3(ix), 1.0 / k1x_big_Q3(ix), 1.45034217834472656) / sx;
+        r.k0 = fma(k0x_big_P3(ix), 1.0 / k0x_big_Q3(ix), 1.0) / sx;
+        r.k1 = fma(k1x_big_P3(ix), 1.0 / k1x_big_Q3(ix), 1.45034217834472656) / sx;
     }
+    return r;
 }
 // large-x series of safe_besselkx (src/eos.jl:27-34)
 __device__ __forceinline__ double besselkx_asym(int nu, double x) {
@@ -120,8 +124,8 @@ __device__ __forceinline__ double besselkx_asym(int nu, double x) {
 __device__ __forceinline__ double safe_besselkx(int nu, double x) {
     if (!(isf(x) && x > 0.0)) return 0.0;
     if (x >= ASYM_X) return besselkx_asym(nu, x);
-    double km, k;
-    besselk01x(x, &km, &k);
+    const K01 b = besselk01x(x);
+    double km = b.k0, k = b.k1;
     if (nu == 0) return km;
     for (int n = 1; n < nu; ++n) { double kp = km + (2.0 * n / x) * k; km = k; k = kp; }
     return k;
@@ -130,17 +134,16 @@ __device__ __forceinline__ double safe_besselkx(int nu, double x) {
 __device__ __forceinline__ void besselk12x(double x, double* k1, double* k2) {
     if (!(isf(x) && x > 0.0)) { *k1 = 0.0; *k2 = 0.0; return; }
     if (x >= ASYM_X) { *k1 = besselkx_asym(1, x); *k2 = besselkx_asym(2, x); return; }
-    double k0;
-    besselk01x(x, &k0, k1);
-    *k2 = k0 + (2.0 * 1 / x) * *k1;
+    const K01 b = besselk01x(x);
+    *k1 = b.k1;
+    *k2 = b.k0 + (2.0 * 1 / x) * b.k1;
 }
 // safe_besselk2x_2d (src2d/bessel2d.jl:111-115)
 __device__ __forceinline__ double besselk2x_fast(double x) {
     if (!(isf(x) && x > 0.0)) return 0.0;
     if (x >= ASYM_X) return besselkx_asym(2, x);
-    double k0, k1;
-    besselk01x(x, &k0, &k1);
-    return k0 + (2 / x) * k1;
+    const K01 b = besselk01x(x);
+    return b.k0 + (2 / x) * b.k1;
 }
 
 // ---- EOS constants (host-prepared, same arithmetic as the reference) ----------
@@ -291,20 +294,74 @@ struct Diss { double nux, nuy, Pi, pixx, pixy, piyy, pieta; };
 // ---- primitive recovery (src2d/primrec2d.jl:300-742) --------------------------
 struct PrimRes { double T, mu, ux, uy, n, e, P; int ok, reason, iters; };
 
-template <int EK>
+// Group-cooperative form: GW consecutive lanes of a wavefront solve one cell.
+// Every lane of a group holds the same iterate; the lanes split the independent
+// evaluations (finite-difference columns, line-search trials, bisection
+// midpoints) and pick results with shuffles, so the iterate sequence is the
+// sequential one, bit for bit.
+constexpr int GW = 8;
+__device__ __forceinline__ int glane() { return threadIdx.x & (GW - 1); }
+__device__ __forceinline__ double gshfl(double v, int src) { return __shfl(v, src, GW); }
+__device__ __forceinline__ unsigned gbits(bool pred) {   // pred of the group's lanes, bit l = lane l
+    const unsigned long long b = __ballot(pred);
+    return (unsigned)((b >> ((threadIdx.x & 63) & ~(GW - 1))) & ((1u << GW) - 1));
+}
+__device__ __forceinline__ double sel8(int k, double a0, double a1, double a2, double a3, double a4, double a5,
+                                       double a6) {
+    double v = a6;
+    v = k == 5 ? a5 : v; v = k == 4 ? a4 : v; v = k == 3 ? a3 : v;
+    v = k == 2 ? a2 : v; v = k == 1 ? a1 : v; v = k == 0 ? a0 : v;
+    return v;
+}
+
+// `nsteps` steps of  mid = (lo+hi)/2; pred(mid) ? hi = mid : lo = mid.
+// COOP: three steps per round; lanes 0..6 take the 7 midpoints of the 3-level
+// tree below [lo, hi] (node n has children 2n+1 for pred true, 2n+2 otherwise).
+template <bool COOP, class Pred>
+__device__ __forceinline__ void bisect(double& lo, double& hi, int nsteps, Pred pred) {
+    if (!COOP) {
+        for (int it = 0; it < nsteps; ++it) {
+            double mid = 0.5 * (lo + hi);
+            if (pred(mid)) hi = mid; else lo = mid;
+        }
+        return;
+    }
+    const int l = glane();
+    for (int done = 0; done < nsteps;) {
+        const int L = nsteps - done < 3 ? nsteps - done : 3;
+        const double m0 = 0.5 * (lo + hi);
+        const double m1 = 0.5 * (lo + m0), m2 = 0.5 * (m0 + hi);
+        const double m3 = 0.5 * (lo + m1), m4 = 0.5 * (m1 + m0), m5 = 0.5 * (m0 + m2), m6 = 0.5 * (m2 + hi);
+        const int nn = (1 << L) - 1;
+        bool pl = false;
+        if (l < nn) pl = pred(sel8(l, m0, m1, m2, m3, m4, m5, m6));
+        const unsigned b = gbits(pl);
+        int node = 0;
+        for (int lev = 0; lev < L; ++lev) {
+            const double mid = sel8(node, m0, m1, m2, m3, m4, m5, m6);
+            const bool p = (b >> node) & 1u;
+            if (p) hi = mid; else lo = mid;
+            node = 2 * node + (p ? 1 : 2);
+        }
+        done += L;
+    }
+}
+
+// _invert_e_for_T (src2d/primrec2d.jl:300-314): 60 bisection steps on log T
+template <int EK, bool COOP>
 __device__ __forceinline__ double invert_e_for_T(double e_target, double phi, const EosP& q) {
     double lo = log(T_MIN), hi = log(T_SOLVE_MAX);
     if (e_target <= 0.0) return lo;
-    for (int it = 0; it < 60; ++it) {
-        double mid = 0.5 * (lo + hi);
+    bisect<COOP>(lo, hi, 60, [&](double mid) {
         double Tm = exp(mid), P, em;
         eos_Pe_2d<EK>(Tm, q.m + Tm * phi, q, &P, &em);
-        if (!isf(em) || em > e_target) hi = mid; else lo = mid;
-    }
+        return !isf(em) || em > e_target;
+    });
     return 0.5 * (lo + hi);
 }
-template <int EK>
-__device__ __noinline__ void seed_from_conserved(double E, double Sx, double Sy, double phi, const EosP& q,
+// seed_from_conserved_2d (src2d/primrec2d.jl:323-347)
+template <int EK, bool COOP>
+__device__ __forceinline__ void seed_from_conserved(double E, double Sx, double Sy, double phi, const EosP& q,
                                                  double* yT_o, double* ux_o, double* uy_o) {
     double M = sqrt(Sx * Sx + Sy * Sy);
     if (!isf(E) || E <= 0.0) { *yT_o = log(0.15); *ux_o = 0.0; *uy_o = 0.0; return; }
@@ -314,7 +371,7 @@ __device__ __noinline__ void seed_from_conserved(double E, double Sx, double Sy,
         v = denom > TINY ? jmin(M / denom, 0.999999) : 0.0;
         double e = denom * (1 - v * v) - P;
         if (e <= 0.0) e = jmax(E, TINY);
-        yT = invert_e_for_T<EK>(e, phi, q);
+        yT = invert_e_for_T<EK, COOP>(e, phi, q);
         double T = exp(yT), Pn, en;
         eos_Pe_2d<EK>(T, q.m + T * phi, q, &Pn, &en);
         if (!isf(Pn)) break;
@@ -329,31 +386,34 @@ __device__ __noinline__ void seed_from_conserved(double E, double Sx, double Sy,
     *uy_o = jclamp(uy, -U_CAP, U_CAP);
 }
 
-// evalF_2d! (src2d/primrec2d.jl:360-390), with the EOS T-part of T = exp(x0)
-// given. Writes F (all Inf when the state or the EOS is non-finite, like the
-// reference) and returns whether every component is finite.
+// conserved inputs of one cell and the Newton row scales
+struct CellIn {
+    double D, Sx, Sy, E;
+    Diss d;
+    double rs[4];
+};
+
+// evalF_2d! (src2d/primrec2d.jl:360-390) at x with T = exp(x0) and the EOS
+// T-part of max(T, T_MIN) given. Writes F (all Inf when the state or the EOS is
+// non-finite, like the reference) and returns whether every component is finite.
 template <int EK, bool NCH>
-__device__ __forceinline__ bool evalF(double F[4], double x0, double x1, double x2, double x3, const TPart& tp,
-                                      double D, double Sx, double Sy, double E, const Diss& d, const EosP& q) {
+__device__ __forceinline__ bool evalF(double F[4], double x0, double x1, double x2, double x3, double T,
+                                      const TPart& tp, const CellIn& c, const EosP& q) {
     if (!(isf(x0) && isf(x1) && isf(x2) && isf(x3))) { F[0] = F[1] = F[2] = F[3] = dinf(); return false; }
-    double T = exp(x0);
     double mu = q.m + T * x1;
     double ut = sqrt(1 + x2 * x2 + x3 * x3);
     double P, n, e;
     eos_assemble<EK>(tp, mu, q, &P, &n, &e);
     if (!(isf(P) && isf(n) && isf(e))) { F[0] = F[1] = F[2] = F[3] = dinf(); return false; }
+    const Diss& d = c.d;
     double weff = e + P + d.Pi, Ptot = P + d.Pi;
     double tt, tx, ty;
     shear_tau_row(x2, x3, ut, d.pixx, d.pixy, d.piyy, &tt, &tx, &ty);
-    F[0] = NCH ? (n * ut + nu_tau(x2, x3, ut, d.nux, d.nuy) - D) : 0.0;
-    F[1] = weff * ut * x2 + tx - Sx;
-    F[2] = weff * ut * x3 + ty - Sy;
-    F[3] = weff * ut * ut - Ptot + tt - E;
+    F[0] = NCH ? (n * ut + nu_tau(x2, x3, ut, d.nux, d.nuy) - c.D) : 0.0;
+    F[1] = weff * ut * x2 + tx - c.Sx;
+    F[2] = weff * ut * x3 + ty - c.Sy;
+    F[3] = weff * ut * ut - Ptot + tt - c.E;
     return isf(F[0]) && isf(F[1]) && isf(F[2]) && isf(F[3]);
-}
-template <int EK>
-__device__ __forceinline__ TPart tpart_of_yT(double yT, const EosP& q) {
-    return eos_tpart<EK>(jmax(exp(yT), T_MIN), q);
 }
 
 // solve_gauss_n! (src2d/primrec2d.jl:33-87), n = N, fully unrolled on registers
@@ -415,160 +475,306 @@ __device__ __forceinline__ bool solve_gauss(double J[4][4], const double F[4], d
 }
 
 // _solve_phi_bisect (src2d/primrec2d.jl:416-434) at the (clamped) final T
-template <int EK>
-__device__ __noinline__ bool solve_phi_bisect(double D, double ux, double uy, double nux, double nuy,
+template <int EK, bool COOP>
+__device__ __forceinline__ bool solve_phi_bisect(double D, double ux, double uy, double nux, double nuy,
                                               const TPart& tp, const EosP& q, double* phi_o) {
     double ut = sqrt(1 + ux * ux + uy * uy);
     double target = D - nu_tau(ux, uy, ut, nux, nuy);
     if (!isf(target) || target <= 0.0) { *phi_o = -PHI_CAP; return false; }
     double lo = -PHI_CAP, hi = PHI_CAP, P, nm, e;
-    for (int it = 0; it < 200; ++it) {
-        double mid = 0.5 * (lo + hi);
-        eos_assemble<EK>(tp, q.m + tp.T * mid, q, &P, &nm, &e);
-        if (isf(nm) && nm * ut > target) hi = mid; else lo = mid;
-    }
+    bisect<COOP>(lo, hi, 200, [&](double mid) {
+        double Pb, nb, eb;
+        eos_assemble<EK>(tp, q.m + tp.T * mid, q, &Pb, &nb, &eb);
+        return isf(nb) && nb * ut > target;
+    });
     double phi = 0.5 * (lo + hi);
     eos_assemble<EK>(tp, q.m + tp.T * phi, q, &P, &nm, &e);
     *phi_o = phi;
     return isf(nm);
 }
 
-// _cons_to_prim_attempt_2d! (src2d/primrec2d.jl:511-742). NCH: eos_has_charge_2d.
-// SEED = false instantiates the warm-started solve only: the caller guarantees a
-// finite warm start (or a vacuum state), so the cold-seed code is not compiled in.
-template <int EK, bool NCH, bool SEED = true>
-__device__ __forceinline__ PrimRes cons_to_prim_attempt(double D, double Sx, double Sy, double E, const Diss& d,
-                                                     double yT0, double phi0, double ux0, double uy0,
-                                                     const EosP& q) {
-    constexpr int NU = NCH ? 4 : 3;
-    constexpr double tol = 1e-12;
-    constexpr int maxit = 80;
-    const double yT_lo = log(T_MIN), yT_hi = log(T_SOLVE_MAX);
-    PrimRes R;
-    R.reason = PRR_UNSET; R.iters = 0;
-    if (!isf(E) || E <= E_VAC || !isf(D) || D < 0.0) {
-        double P, n, e;
-        eos_Pne_2d<EK>(T_MIN, 0.0, q, &P, &n, &e);   // eos_Pne at T_MIN: identical branch
-        R = {T_MIN, 0.0, 0.0, 0.0, 0.0, jmax(E, 0.0), P, 0, PRR_VACUUM, 0};
-        return R;
-    }
+// The Newton iterate of _cons_to_prim_attempt_2d!. `it` is the reference's loop
+// variable: an iterate can stop before iteration `it` and resume from (x, it),
+// since T, the T-part and F are pure functions of x.
+struct NewtonState {
     double x[4];
-    x[1] = (NCH && isf(phi0)) ? jclamp(phi0, -PHI_CAP, PHI_CAP) : 0.0;
-    if (isf(yT0) && isf(ux0) && isf(uy0)) {
-        x[0] = jclamp(yT0, yT_lo, yT_hi);
-        x[2] = jclamp(ux0, -U_CAP, U_CAP);
-        x[3] = jclamp(uy0, -U_CAP, U_CAP);
-    } else if (SEED) {
-        double sy, sx, syy;
-        seed_from_conserved<EK>(E, Sx, Sy, x[1], q, &sy, &sx, &syy);
-        x[0] = jclamp(sy, yT_lo, yT_hi); x[2] = sx; x[3] = syy;
-    }
-    TPart tp = tpart_of_yT<EK>(x[0], q);
-    double F[4];
-    if (!evalF<EK, NCH>(F, x[0], x[1], x[2], x[3], tp, D, Sx, Sy, E, d, q)) {
-        double T = jclamp(exp(x[0]), T_MIN, T_SOLVE_MAX), P, n, e;
-        eos_Pne_2d<EK>(T, q.m, q, &P, &n, &e);
-        R = {T, q.m, 0.0, 0.0, 0.0, jmax(e, 0.0), P, 0, PRR_NONFINITE_INITIAL, 0};
-        return R;
-    }
+    double Tr;      // exp(x[0])
+    TPart tp;       // EOS T-part at max(Tr, T_MIN)
+    double F[4];    // residual at x
+    int it, iters, reason;
+    bool converged;
+};
+constexpr int NEWTON_MAXIT = 80;
+__device__ __forceinline__ bool newton_paused(const NewtonState& s) {
+    return !s.converged && s.reason == PRR_UNSET && s.it <= NEWTON_MAXIT;
+}
+
+template <bool NCH>
+__device__ __forceinline__ CellIn make_cell(double D, double Sx, double Sy, double E, const Diss& d) {
+    CellIn c;
+    c.D = D; c.Sx = Sx; c.Sy = Sy; c.E = E; c.d = d;
     const double sD = jmax(fabs(D), 1e-12);
     const double sS = jmax(jmax(jmax(fabs(Sx), fabs(Sy)), 1e-10 * fabs(E)), 1e-12);
     const double sE = jmax(fabs(E), 1e-12);
-    const double rs[4] = {1 / sD, 1 / sS, 1 / sS, 1 / sE};
-    constexpr int rlo = NCH ? 0 : 1;
-    bool converged = false;
-    int reason = PRR_UNSET, iters = 0;
-    for (int it = 1; it <= maxit; ++it) {
-        double nrm = 0.0;
-#pragma unroll
-        for (int r = rlo; r < 4; ++r) nrm = jmax(nrm, fabs(F[r] * rs[r]));
-        if (nrm < tol) { converged = true; reason = PRR_CONVERGED; iters = it - 1; break; }
-        double J[4][4];
-        bool okj = true;
+    c.rs[0] = 1 / sD; c.rs[1] = 1 / sS; c.rs[2] = 1 / sS; c.rs[3] = 1 / sE;
+    return c;
+}
+
+// set the iterate to x and evaluate T, the T-part and F there
+template <int EK, bool NCH>
+__device__ __forceinline__ bool newton_set(NewtonState& s, const CellIn& c, const EosP& q) {
+    s.Tr = exp(s.x[0]);
+    s.tp = eos_tpart<EK>(jmax(s.Tr, T_MIN), q);
+    return evalF<EK, NCH>(s.F, s.x[0], s.x[1], s.x[2], s.x[3], s.Tr, s.tp, c, q);
+}
+
+// head of _cons_to_prim_attempt_2d! (src2d/primrec2d.jl:511-590): vacuum test,
+// warm start or cold seed, first residual. Returns true with the final R when
+// the attempt ends here (vacuum, non-finite start). SEED = false: the caller
+// guarantees a finite warm start or a vacuum state.
+template <int EK, bool NCH, bool SEED, bool COOP>
+__device__ __forceinline__ bool newton_begin(const CellIn& c, double yT0, double phi0, double ux0, double uy0,
+                                             const EosP& q, NewtonState& s, PrimRes& R) {
+    const double yT_lo = log(T_MIN), yT_hi = log(T_SOLVE_MAX);
+    if (!isf(c.E) || c.E <= E_VAC || !isf(c.D) || c.D < 0.0) {
+        double P, n, e;
+        eos_Pne_2d<EK>(T_MIN, 0.0, q, &P, &n, &e);   // eos_Pne at T_MIN: identical branch
+        R = {T_MIN, 0.0, 0.0, 0.0, 0.0, jmax(c.E, 0.0), P, 0, PRR_VACUUM, 0};
+        return true;
+    }
+    s.x[1] = (NCH && isf(phi0)) ? jclamp(phi0, -PHI_CAP, PHI_CAP) : 0.0;
+    if (isf(yT0) && isf(ux0) && isf(uy0)) {
+        s.x[0] = jclamp(yT0, yT_lo, yT_hi);
+        s.x[2] = jclamp(ux0, -U_CAP, U_CAP);
+        s.x[3] = jclamp(uy0, -U_CAP, U_CAP);
+    } else if (SEED) {
+        double sy, sx, syy;
+        seed_from_conserved<EK, COOP>(c.E, c.Sx, c.Sy, s.x[1], q, &sy, &sx, &syy);
+        s.x[0] = jclamp(sy, yT_lo, yT_hi); s.x[2] = sx; s.x[3] = syy;
+    }
+    s.it = 1; s.iters = 0; s.reason = PRR_UNSET; s.converged = false;
+    if (!newton_set<EK, NCH>(s, c, q)) {
+        double T = jclamp(s.Tr, T_MIN, T_SOLVE_MAX), P, n, e;
+        eos_Pne_2d<EK>(T, q.m, q, &P, &n, &e);
+        R = {T, q.m, 0.0, 0.0, 0.0, jmax(e, 0.0), P, 0, PRR_NONFINITE_INITIAL, 0};
+        return true;
+    }
+    return false;
+}
+
+// finite-difference Jacobian, scaled rows (src2d/primrec2d.jl:605-632)
+template <int EK, bool NCH, bool COOP>
+__device__ __forceinline__ bool newton_jacobian(const NewtonState& s, const CellIn& c, const EosP& q,
+                                                double J[4][4]) {
+    constexpr int NU = NCH ? 4 : 3;
+    if (!COOP) {
 #pragma unroll
         for (int k = 0; k < NU; ++k) {
-            const int s = NCH ? k : (k == 0 ? 0 : k + 1);
-            double xs = x[s];
+            const int si = NCH ? k : (k == 0 ? 0 : k + 1);
+            double xs = s.x[si];
             double h = 1e-6 * jmax(1.0, fabs(xs));
             double Fp[4], Fm[4];
-            double xp[4] = {x[0], x[1], x[2], x[3]}, xm[4] = {x[0], x[1], x[2], x[3]};
-            xp[s] = xs + h; xm[s] = xs - h;
+            double xp[4] = {s.x[0], s.x[1], s.x[2], s.x[3]}, xm[4] = {s.x[0], s.x[1], s.x[2], s.x[3]};
+            xp[si] = xs + h; xm[si] = xs - h;
             bool a, b;
-            if (s == 0) {
-                TPart tpp = tpart_of_yT<EK>(xp[0], q);
-                TPart tpm = tpart_of_yT<EK>(xm[0], q);
-                a = evalF<EK, NCH>(Fp, xp[0], xp[1], xp[2], xp[3], tpp, D, Sx, Sy, E, d, q);
-                b = evalF<EK, NCH>(Fm, xm[0], xm[1], xm[2], xm[3], tpm, D, Sx, Sy, E, d, q);
+            if (si == 0) {
+                const double Tp = exp(xp[0]), Tm = exp(xm[0]);
+                TPart tpp = eos_tpart<EK>(jmax(Tp, T_MIN), q);
+                TPart tpm = eos_tpart<EK>(jmax(Tm, T_MIN), q);
+                a = evalF<EK, NCH>(Fp, xp[0], xp[1], xp[2], xp[3], Tp, tpp, c, q);
+                b = evalF<EK, NCH>(Fm, xm[0], xm[1], xm[2], xm[3], Tm, tpm, c, q);
             } else {
-                a = evalF<EK, NCH>(Fp, xp[0], xp[1], xp[2], xp[3], tp, D, Sx, Sy, E, d, q);
-                b = evalF<EK, NCH>(Fm, xm[0], xm[1], xm[2], xm[3], tp, D, Sx, Sy, E, d, q);
+                a = evalF<EK, NCH>(Fp, xp[0], xp[1], xp[2], xp[3], s.Tr, s.tp, c, q);
+                b = evalF<EK, NCH>(Fm, xm[0], xm[1], xm[2], xm[3], s.Tr, s.tp, c, q);
             }
-            if (!(a && b)) { okj = false; break; }
+            if (!(a && b)) return false;
             double inv2h = 1 / (2 * h);
 #pragma unroll
             for (int r = 0; r < NU; ++r) {
                 const int rr = NCH ? r : r + 1;
-                J[r][k] = ((Fp[rr] - Fm[rr]) * inv2h) * rs[rr];
+                J[r][k] = ((Fp[rr] - Fm[rr]) * inv2h) * c.rs[rr];
             }
         }
-        if (!okj) { reason = PRR_NONFINITE_JACOBIAN; break; }
+        return true;
+    }
+    // lane l evaluates column k = l/2 at x + h e_k (l even) or x - h e_k (l odd)
+    const int l = glane();
+    const int k = l >> 1;
+    const bool act = k < NU;
+    const int si = NCH ? k : (k == 0 ? 0 : k + 1);
+    const double xs = si == 0 ? s.x[0] : si == 1 ? s.x[1] : si == 2 ? s.x[2] : s.x[3];
+    const double h = 1e-6 * jmax(1.0, fabs(xs));
+    const double xv = (l & 1) ? xs - h : xs + h;
+    double y[4] = {s.x[0], s.x[1], s.x[2], s.x[3]};
+    y[0] = si == 0 ? xv : y[0];
+    y[1] = si == 1 ? xv : y[1];
+    y[2] = si == 2 ? xv : y[2];
+    y[3] = si == 3 ? xv : y[3];
+    double Tl = s.Tr;
+    TPart tl = s.tp;
+    if (act && si == 0) { Tl = exp(y[0]); tl = eos_tpart<EK>(jmax(Tl, T_MIN), q); }
+    double Fl[4] = {0.0, 0.0, 0.0, 0.0};
+    bool okl = true;
+    if (act) okl = evalF<EK, NCH>(Fl, y[0], y[1], y[2], y[3], Tl, tl, c, q);
+    if (gbits(!okl) != 0u) return false;
+    // even lane 2k forms column k from its partner's F(x - h e_k)
+    const double inv2h = 1 / (2 * h);
+    double col[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const double Fm = __shfl_xor(Fl[r], 1, GW);
+        col[r] = ((Fl[r] - Fm) * inv2h) * c.rs[r];
+    }
+#pragma unroll
+    for (int kk = 0; kk < NU; ++kk)
+#pragma unroll
+        for (int r = 0; r < NU; ++r) {
+            const int rr = NCH ? r : r + 1;
+            J[r][kk] = gshfl(col[rr], 2 * kk);
+        }
+    return true;
+}
+
+// the line search of src2d/primrec2d.jl:648-691: trial ls at alpha 2^-ls,
+// accepted at the first trial with a finite residual below (1 - 1e-4 a) nrm.
+// On return x is the accepted trial, or the last one when none is.
+template <int EK, bool NCH, bool COOP>
+__device__ __forceinline__ bool newton_linesearch(NewtonState& s, const CellIn& c, const EosP& q,
+                                                  const double delta[4], double alpha0, double nrm) {
+    constexpr int NU = NCH ? 4 : 3;
+    constexpr int rlo = NCH ? 0 : 1;
+    const double yT_lo = log(T_MIN), yT_hi = log(T_SOLVE_MAX);
+    const double xs0 = s.x[0], xs1 = s.x[1], xs2 = s.x[2], xs3 = s.x[3];
+    auto trial_point = [&](double alpha, double t[4]) {
+        t[0] = xs0; t[1] = xs1; t[2] = xs2; t[3] = xs3;
+#pragma unroll
+        for (int k = 0; k < NU; ++k) {
+            const int si = NCH ? k : (k == 0 ? 0 : k + 1);
+            t[si] += alpha * delta[k];
+        }
+        t[0] = jclamp(t[0], yT_lo, yT_hi);
+        if (NCH) t[1] = jclamp(t[1], -PHI_CAP, PHI_CAP);
+        t[2] = jclamp(t[2], -U_CAP, U_CAP);
+        t[3] = jclamp(t[3], -U_CAP, U_CAP);
+    };
+    if (!COOP) {
+        double alpha = alpha0;
+        for (int ls = 0; ls < 12; ++ls) {
+            double t[4];
+            trial_point(alpha, t);
+            s.x[0] = t[0]; s.x[1] = t[1]; s.x[2] = t[2]; s.x[3] = t[3];
+            s.Tr = exp(t[0]);
+            const double Tn = jmax(s.Tr, T_MIN);
+            if (Tn != s.tp.T) s.tp = eos_tpart<EK>(Tn, q);
+            if (evalF<EK, NCH>(s.F, t[0], t[1], t[2], t[3], s.Tr, s.tp, c, q)) {
+                double trial = 0.0;
+#pragma unroll
+                for (int r = rlo; r < 4; ++r) trial = jmax(trial, fabs(s.F[r] * c.rs[r]));
+                if (trial < nrm * (1 - 1e-4 * alpha)) return true;
+            }
+            alpha *= 0.5;
+        }
+        return false;
+    }
+    // trials 0..7 on lanes 0..7, then 8..11 on lanes 0..3
+    const int l = glane();
+    for (int round = 0; round < 2; ++round) {
+        const int ls = 8 * round + l;
+        const bool act = ls < 12;
+        double alpha = alpha0;
+        for (int j = 0; j < ls; ++j) alpha *= 0.5;
+        double t[4];
+        trial_point(alpha, t);
+        const double Tl = exp(t[0]);
+        const double Tn = jmax(Tl, T_MIN);
+        TPart tl = s.tp;
+        double Fl[4] = {0.0, 0.0, 0.0, 0.0};
+        bool accl = false;
+        if (act) {
+            if (Tn != s.tp.T) tl = eos_tpart<EK>(Tn, q);
+            if (evalF<EK, NCH>(Fl, t[0], t[1], t[2], t[3], Tl, tl, c, q)) {
+                double trial = 0.0;
+#pragma unroll
+                for (int r = rlo; r < 4; ++r) trial = jmax(trial, fabs(Fl[r] * c.rs[r]));
+                accl = trial < nrm * (1 - 1e-4 * alpha);
+            }
+        }
+        const unsigned b = gbits(accl);
+        const bool last = round == 1;
+        if (b != 0u || last) {
+            const int src = b != 0u ? __builtin_ctz(b) : 3;   // trial 11 is lane 3 of round 1
+#pragma unroll
+            for (int r = 0; r < 4; ++r) { s.x[r] = gshfl(t[r], src); s.F[r] = gshfl(Fl[r], src); }
+            s.Tr = gshfl(Tl, src);
+            s.tp.T = gshfl(tl.T, src); s.tp.Pl = gshfl(tl.Pl, src); s.tp.el = gshfl(tl.el, src);
+            s.tp.K2 = gshfl(tl.K2, src); s.tp.K1 = gshfl(tl.K1, src); s.tp.x = gshfl(tl.x, src);
+            return b != 0u;
+        }
+    }
+    return false;
+}
+
+// Newton iterations it .. itcap of _cons_to_prim_attempt_2d! (src2d/primrec2d.jl:591-692)
+template <int EK, bool NCH, bool COOP>
+__device__ __forceinline__ void newton_iterate(NewtonState& s, const CellIn& c, const EosP& q, int itcap) {
+    constexpr int NU = NCH ? 4 : 3;
+    constexpr int rlo = NCH ? 0 : 1;
+    constexpr double tol = 1e-12;
+    for (; s.it <= NEWTON_MAXIT; ++s.it) {
+        if (s.it > itcap) return;                    // paused before iteration s.it
+        double nrm = 0.0;
+#pragma unroll
+        for (int r = rlo; r < 4; ++r) nrm = jmax(nrm, fabs(s.F[r] * c.rs[r]));
+        if (nrm < tol) { s.converged = true; s.reason = PRR_CONVERGED; s.iters = s.it - 1; return; }
+        double J[4][4];
+        if (!newton_jacobian<EK, NCH, COOP>(s, c, q, J)) { s.reason = PRR_NONFINITE_JACOBIAN; return; }
         double Fact[4], delta[4];
 #pragma unroll
         for (int r = 0; r < NU; ++r) {
             const int rr = NCH ? r : r + 1;
-            Fact[r] = F[rr] * rs[rr];
+            Fact[r] = s.F[rr] * c.rs[rr];
             delta[r] = 0.0;
         }
-        if (!solve_gauss<NU>(J, Fact, delta)) { reason = PRR_LINSOLVE_FAILED; break; }
+        if (!solve_gauss<NU>(J, Fact, delta)) { s.reason = PRR_LINSOLVE_FAILED; return; }
         double alpha = 1.0;
 #pragma unroll
         for (int k = 0; k < NU; ++k) { double ad = fabs(delta[k]); if (ad > 1.0) alpha = jmin(alpha, 1.0 / ad); }
-        const double xs0 = x[0], xs1 = x[1], xs2 = x[2], xs3 = x[3];
-        bool accepted = false;
-        for (int ls = 0; ls < 12; ++ls) {
-            double t[4] = {xs0, xs1, xs2, xs3};
-#pragma unroll
-            for (int k = 0; k < NU; ++k) {
-                const int s = NCH ? k : (k == 0 ? 0 : k + 1);
-                t[s] += alpha * delta[k];
-            }
-            t[0] = jclamp(t[0], yT_lo, yT_hi);
-            if (NCH) t[1] = jclamp(t[1], -PHI_CAP, PHI_CAP);
-            t[2] = jclamp(t[2], -U_CAP, U_CAP);
-            t[3] = jclamp(t[3], -U_CAP, U_CAP);
-            x[0] = t[0]; x[1] = t[1]; x[2] = t[2]; x[3] = t[3];
-            double Tn = jmax(exp(t[0]), T_MIN);
-            if (Tn != tp.T) tp = eos_tpart<EK>(Tn, q);
-            if (evalF<EK, NCH>(F, t[0], t[1], t[2], t[3], tp, D, Sx, Sy, E, d, q)) {
-                double trial = 0.0;
-#pragma unroll
-                for (int r = rlo; r < 4; ++r) trial = jmax(trial, fabs(F[r] * rs[r]));
-                if (trial < nrm * (1 - 1e-4 * alpha)) { accepted = true; break; }
-            }
-            alpha *= 0.5;
+        if (!newton_linesearch<EK, NCH, COOP>(s, c, q, delta, alpha, nrm)) {
+            s.reason = PRR_LINESEARCH_FAILED;
+            return;
         }
-        if (!accepted) { reason = PRR_LINESEARCH_FAILED; break; }
-        if (it == maxit) iters = maxit;
+        if (s.it == NEWTON_MAXIT) s.iters = NEWTON_MAXIT;
     }
-    double T = jclamp(exp(x[0]), T_MIN, T_SOLVE_MAX);
-    double mu = NCH ? (q.m + T * x[1]) : q.m;
-    double ux = x[2], uy = x[3];
-    TPart tf = (jmax(T, T_MIN) == tp.T) ? tp : eos_tpart<EK>(jmax(T, T_MIN), q);
+}
+
+// tail of _cons_to_prim_attempt_2d! (src2d/primrec2d.jl:694-742): final state,
+// residual acceptance, staged phi fallback. BISECT = false: returns with
+// reason PRR_UNSET where the staged fallback would start, leaving it to a caller
+// that resumes the tail with BISECT = true from the same iterate.
+template <int EK, bool NCH, bool COOP, bool BISECT = true>
+__device__ __forceinline__ PrimRes newton_finish(NewtonState& s, const CellIn& c, const EosP& q) {
+    constexpr int rlo = NCH ? 0 : 1;
+    constexpr double tol = 1e-12;
+    PrimRes R;
+    double T = jclamp(s.Tr, T_MIN, T_SOLVE_MAX);
+    double mu = NCH ? (q.m + T * s.x[1]) : q.m;
+    double ux = s.x[2], uy = s.x[3];
+    TPart tf = (jmax(T, T_MIN) == s.tp.T) ? s.tp : eos_tpart<EK>(jmax(T, T_MIN), q);
     double P, n, e;
     eos_assemble<EK>(tf, mu, q, &P, &n, &e);
-    if (!converged) reason = (reason == PRR_UNSET) ? PRR_MAXIT : reason;
     // residual at the final iterate (tp belongs to x[0])
     double rows[4] = {0, 0, 0, 0};
     double resnorm = 0.0;
-    evalF<EK, NCH>(F, x[0], x[1], x[2], x[3], tp, D, Sx, Sy, E, d, q);
+    evalF<EK, NCH>(s.F, s.x[0], s.x[1], s.x[2], s.x[3], s.Tr, s.tp, c, q);
 #pragma unroll
-    for (int r = rlo; r < 4; ++r) { rows[r] = fabs(F[r] * rs[r]); resnorm = jmax(resnorm, rows[r]); }
-    R.iters = iters;
+    for (int r = rlo; r < 4; ++r) { rows[r] = fabs(s.F[r] * c.rs[r]); resnorm = jmax(resnorm, rows[r]); }
+    const int iters = s.iters;
     if (!(isf(resnorm) && resnorm < tol)) {
         double hyd = jmax(jmax(rows[1], rows[2]), rows[3]);
         if (NCH && isf(hyd) && hyd < tol) {
+            if (!BISECT) { R.reason = PRR_UNSET; R.ok = 0; R.iters = iters; return R; }
             double phib;
-            bool okb = solve_phi_bisect<EK>(D, ux, uy, d.nux, d.nuy, tf, q, &phib);
+            bool okb = solve_phi_bisect<EK, COOP>(c.D, ux, uy, c.d.nux, c.d.nuy, tf, q, &phib);
             double ph = jclamp(phib, -PHI_CAP, PHI_CAP);
             mu = q.m + T * ph;
             eos_assemble<EK>(tf, mu, q, &P, &n, &e);
@@ -579,22 +785,32 @@ __device__ __forceinline__ PrimRes cons_to_prim_attempt(double D, double Sx, dou
         return R;
     }
     R = {T, mu, ux, uy, n, jmax(e, 0.0), P, 1, PRR_CONVERGED, iters};
-    (void)reason;
     return R;
+}
+
+// _cons_to_prim_attempt_2d! (src2d/primrec2d.jl:511-742). NCH: eos_has_charge_2d.
+template <int EK, bool NCH, bool SEED = true, bool COOP = false>
+__device__ __forceinline__ PrimRes cons_to_prim_attempt(const CellIn& c, double yT0, double phi0, double ux0,
+                                                        double uy0, const EosP& q) {
+    NewtonState s;
+    PrimRes R;
+    if (newton_begin<EK, NCH, SEED, COOP>(c, yT0, phi0, ux0, uy0, q, s, R)) return R;
+    newton_iterate<EK, NCH, COOP>(s, c, q, NEWTON_MAXIT);
+    return newton_finish<EK, NCH, COOP>(s, c, q);
 }
 
 // cons_to_prim_2d! (src2d/primrec2d.jl:465-486): one fresh-seed retry. Written
 // as a two-pass loop so the Newton body is instantiated once.
 template <int EK, bool NCH>
-__device__ __forceinline__ PrimRes cons_to_prim(double D, double Sx, double Sy, double E, const Diss& d,
-                                                double yT0, double phi0, double ux0, double uy0, const EosP& q) {
+__device__ __forceinline__ PrimRes cons_to_prim(const CellIn& c, double yT0, double phi0, double ux0, double uy0,
+                                                const EosP& q) {
     PrimRes r;
     int its = 0;
     const bool warm = isf(yT0) && isf(ux0) && isf(uy0);
     const double nan = __longlong_as_double(0x7ff8000000000000LL);
 #pragma unroll 1
     for (int pass = 0; pass < 2; ++pass) {
-        r = cons_to_prim_attempt<EK, NCH, true>(D, Sx, Sy, E, d, yT0, phi0, ux0, uy0, q);
+        r = cons_to_prim_attempt<EK, NCH, true>(c, yT0, phi0, ux0, uy0, q);
         its += r.iters;
         if (r.ok || r.reason == PRR_VACUUM || !warm) break;
         yT0 = nan; phi0 = nan; ux0 = nan; uy0 = nan;

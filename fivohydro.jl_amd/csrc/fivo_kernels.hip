@@ -80,12 +80,17 @@ __global__ __launch_bounds__(256) void k_bc(double* __restrict__ U, Params p, in
 }
 
 // ---- primitive pass (src2d/rhs2d.jl:67-160) ---------------------------------------
-// Two kernels. k_prim runs every cell whose Newton starts warm (the production
-// path) or is vacuum; a cell with no warm start, or whose warm solve failed (the
-// fresh-seed retry of cons_to_prim_2d!, src2d/primrec2d.jl:465-486), is appended
-// to a worklist that k_prim_cold solves from the cold seed. The cold seed costs
-// ~180 EOS calls; kept out of k_prim it no longer stalls the 63 warm lanes of its
-// wavefront, and k_prim carries no seed code (fewer registers).
+// Two kernels. k_prim gives every cell whose Newton starts warm (the production
+// path) or is vacuum PRIM_WARM_ITCAP iterations, one lane per cell; almost all
+// cells converge there. A cell still iterating is paused (its iterate kept in
+// the warm-start arrays) and queued; a cell with no warm start, or whose warm
+// solve failed (the fresh-seed retry of cons_to_prim_2d!,
+// src2d/primrec2d.jl:465-486), is queued for the cold seed. k_prim_coop then
+// solves the queue with GW lanes per cell: the slow cells set the pass's
+// latency, and cooperating lanes cut each iteration to about two residual
+// evaluations on the critical path instead of nine.
+constexpr int PRIM_WARM_ITCAP = 6;
+constexpr int PRIM_JOB_TAIL = 1000;   // job.it = PRIM_JOB_TAIL + iters: only the staged fallback is left
 __device__ __forceinline__ void prim_finalize(const Params& p, const DevWork& w, int i, const PrimRes& r0,
                                               const Diss& d, unsigned long long& nfail, unsigned long long& nvac) {
     double T = r0.T, mu = r0.mu, ux = r0.ux, uy = r0.uy, n = r0.n, e = r0.e, P = r0.P;
@@ -147,45 +152,89 @@ __device__ __forceinline__ void prim_counts(DevScalars* sc, unsigned long long i
 
 template <int EK, bool NCH>
 __global__ __launch_bounds__(256) void k_prim(const double* __restrict__ U, Params p, double tau, DevWork w,
-                                              DevScalars* sc, int* coldlist) {
+                                              DevScalars* sc, PrimJob* jobs) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long its = 0, nfail = 0, nvac = 0;
     if (i < p.Ntot) {
         const double D = safe_div(U[i], tau), Sx = U[P_ + i], Sy = U[2 * P_ + i], E = U[3 * P_ + i];
-        const Diss d = load_diss(U, p, i);
+        const CellIn c = make_cell<NCH>(D, Sx, Sy, E, load_diss(U, p, i));
         const double yT0 = w.x0_yT[i], ux0 = w.x0_ux[i], uy0 = w.x0_uy[i];
         const bool vac = !isf(E) || E <= E_VAC || !isf(D) || D < 0.0;
         const bool warm = isf(yT0) && isf(ux0) && isf(uy0);
-        bool done = true;
+        int job = 0;                                 // < 0: done here; 0: cold seed; > 0: resume
         if (vac || warm) {
-            PrimRes r = cons_to_prim_attempt<EK, NCH, false>(D, Sx, Sy, E, d, yT0, w.x0_phi[i], ux0, uy0, p.eos);
-            its = r.iters;
-            if (r.ok || r.reason == PRR_VACUUM) prim_finalize(p, w, i, r, d, nfail, nvac);
-            else done = false;                       // retry from the fresh seed
-        } else {
-            done = false;
+            NewtonState s;
+            PrimRes r;
+            bool fin = newton_begin<EK, NCH, false, false>(c, yT0, w.x0_phi[i], ux0, uy0, p.eos, s, r);
+            bool paused = false;
+            if (!fin) {
+                newton_iterate<EK, NCH, false>(s, c, p.eos, PRIM_WARM_ITCAP);
+                paused = newton_paused(s);
+                if (paused) {
+                    job = s.it;
+                    w.x0_yT[i] = s.x[0]; w.x0_phi[i] = s.x[1]; w.x0_ux[i] = s.x[2]; w.x0_uy[i] = s.x[3];
+                } else {
+                    r = newton_finish<EK, NCH, false, false>(s, c, p.eos);
+                    if (r.reason == PRR_UNSET) {      // staged phi fallback: done by the group
+                        paused = true;
+                        job = PRIM_JOB_TAIL + s.iters;
+                        w.x0_yT[i] = s.x[0]; w.x0_phi[i] = s.x[1]; w.x0_ux[i] = s.x[2]; w.x0_uy[i] = s.x[3];
+                    }
+                }
+            }
+            if (!paused) {
+                its = r.iters;
+                if (r.ok || r.reason == PRR_VACUUM) { prim_finalize(p, w, i, r, c.d, nfail, nvac); job = -1; }
+            }
         }
-        if (!done) {
+        if (job >= 0) {
             const unsigned long long k = atomicAdd(&sc->ncold, 1ull);
-            coldlist[k] = i;
+            jobs[k] = PrimJob{i, job};
         }
     }
     prim_counts(sc, its, nfail, nvac);
 }
 
 template <int EK, bool NCH>
-__global__ __launch_bounds__(64) void k_prim_cold(const double* __restrict__ U, Params p, double tau, DevWork w,
-                                                  DevScalars* sc, const int* coldlist) {
+__global__ __launch_bounds__(256) void k_prim_coop(const double* __restrict__ U, Params p, double tau, DevWork w,
+                                                   DevScalars* sc, const PrimJob* __restrict__ jobs) {
     const int n = (int)sc->ncold;
+    const int ngroups = gridDim.x * (blockDim.x / GW);
+    const bool lead = glane() == 0;
     unsigned long long its = 0, nfail = 0, nvac = 0;
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
-        const int i = coldlist[t];
+    for (int t = (blockIdx.x * blockDim.x + threadIdx.x) / GW; t < n; t += ngroups) {
+        const PrimJob jb = jobs[t];
+        const int i = jb.cell;
         const double D = safe_div(U[i], tau), Sx = U[P_ + i], Sy = U[2 * P_ + i], E = U[3 * P_ + i];
-        const Diss d = load_diss(U, p, i);
+        const CellIn c = make_cell<NCH>(D, Sx, Sy, E, load_diss(U, p, i));
+        NewtonState s;
+        PrimRes r;
+        int itsum = 0;
+        bool resume = jb.it > 0;                      // else: solve from the fresh seed
         const double nan = __longlong_as_double(0x7ff8000000000000LL);
-        PrimRes r = cons_to_prim_attempt<EK, NCH, true>(D, Sx, Sy, E, d, nan, nan, nan, nan, p.eos);
-        its += r.iters;
-        prim_finalize(p, w, i, r, d, nfail, nvac);
+#pragma unroll 1
+        for (int pass = 0; pass < 2; ++pass) {        // a failed resumed solve retries from the seed
+            bool fin = false;
+            if (resume) {
+                s.x[0] = w.x0_yT[i]; s.x[1] = w.x0_phi[i]; s.x[2] = w.x0_ux[i]; s.x[3] = w.x0_uy[i];
+                s.it = jb.it; s.iters = 0; s.reason = PRR_UNSET; s.converged = false;
+                newton_set<EK, NCH>(s, c, p.eos);
+            } else {
+                fin = newton_begin<EK, NCH, true, true>(c, nan, nan, nan, nan, p.eos, s, r);
+            }
+            if (!fin) {
+                if (resume && jb.it >= PRIM_JOB_TAIL) s.iters = jb.it - PRIM_JOB_TAIL;   // tail only
+                else newton_iterate<EK, NCH, true>(s, c, p.eos, NEWTON_MAXIT);
+                r = newton_finish<EK, NCH, true>(s, c, p.eos);
+            }
+            itsum += r.iters;
+            if (!resume || r.ok || r.reason == PRR_VACUUM) break;
+            resume = false;
+        }
+        if (lead) {
+            its += itsum;
+            prim_finalize(p, w, i, r, c.d, nfail, nvac);
+        }
     }
     prim_counts(sc, its, nfail, nvac);
 }
@@ -447,7 +496,8 @@ template <int EK, bool NCH>
 __device__ __forceinline__ bool admissible_newton(const CellState& s, const Params& p, double tau) {
     const Diss d = {s.nux, s.nuy, s.Pi, s.pixx, s.pixy, s.piyy, s.pieta};
     const double nan = __longlong_as_double(0x7ff8000000000000LL);
-    PrimRes r = cons_to_prim<EK, NCH>(safe_div(s.D, tau), s.Sx, s.Sy, s.E, d, nan, nan, nan, nan, p.eos);
+    PrimRes r = cons_to_prim<EK, NCH>(make_cell<NCH>(safe_div(s.D, tau), s.Sx, s.Sy, s.E, d), nan, nan, nan, nan,
+                                      p.eos);
     return r.ok || r.reason == PRR_VACUUM;
 }
 
@@ -762,10 +812,12 @@ hipError_t launch_prim(Ctx& c, const double* U, double tau) {
     const Params& p = c.p;
     // the worklist counter is cleared with the other per-pass scalars
     EOS_DISPATCH(c, hipLaunchKernelGGL((k_prim<EK, NCH>), dim3(nblk(p.Ntot, 256)), dim3(256), 0, c.stream, U, p,
-                                       tau, c.w, c.sc, c.coldlist));
-    const int nb = std::min(nblk(p.Ntot, 64), 4096);
-    EOS_DISPATCH(c, hipLaunchKernelGGL((k_prim_cold<EK, NCH>), dim3(nb), dim3(64), 0, c.stream, U, p, tau, c.w,
-                                       c.sc, c.coldlist));
+                                       tau, c.w, c.sc, c.jobs));
+    // one group of GW lanes per queued cell; 2048 blocks hold 64k groups, more
+    // than the queue of a warm pass, and stride over the queue of a cold one
+    const int nb = std::min(nblk((int64_t)p.Ntot * GW, 256), 2048);
+    EOS_DISPATCH(c, hipLaunchKernelGGL((k_prim_coop<EK, NCH>), dim3(nb), dim3(256), 0, c.stream, U, p, tau, c.w,
+                                       c.sc, c.jobs));
     return hipGetLastError();
 }
 

@@ -104,8 +104,17 @@ int orc_set_work(Ctx* c, const fivo_work_view* v) {
 int orc_reset_history(Ctx* c) { reset_history(*c); return FIVO_OK; }
 // diagnostics: enable per-cell Newton iteration recording; copy the histogram out
 void orc_debug_iters(Ctx* c, int enable, int64_t* hist256) {
-    if (enable) c->w.dbg_iters.assign(c->g.Ntot, -1); else c->w.dbg_iters.clear();
+    if (enable) { c->w.dbg_iters.assign(c->g.Ntot, -1); c->w.dbg_class.assign(c->g.Ntot, 0); }
+    else { c->w.dbg_iters.clear(); c->w.dbg_class.clear(); }
     if (hist256) for (int k = 0; k < 256; ++k) hist256[k] = k < (int)c->w.dbg_hist.size() ? c->w.dbg_hist[k] : 0;
+}
+void orc_debug_hist2(Ctx* c, int64_t* h) {
+    for (int k = 0; k < 2 * 8 * 256; ++k) h[k] = k < (int)c->w.dbg_hist2.size() ? c->w.dbg_hist2[k] : 0;
+}
+// per-cell iterations and class of the last primitive pass (recording enabled)
+void orc_debug_cells(Ctx* c, int32_t* iters, int32_t* cls) {
+    const size_t n = c->w.dbg_iters.size();
+    for (size_t i = 0; i < n; ++i) { iters[i] = c->w.dbg_iters[i]; cls[i] = c->w.dbg_class[i]; }
 }
 
 // finalize_ic! (main2D.jl:164-171)

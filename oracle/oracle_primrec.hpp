@@ -257,15 +257,19 @@ static inline PrimOut cons_to_prim_attempt(const Cons& c, const Diss& d, double 
 // cons_to_prim_2d! (src2d/primrec2d.jl:465-486): one fresh-seed retry
 static inline PrimOut cons_to_prim(const Cons& c, const Diss& d, double tau, const EOS& q,
                                    double yT0, double phi0, double ux0, double uy0, PrimStat& st,
-                                   int* iters_total = nullptr) {
+                                   int* iters_total = nullptr, int* first_iters = nullptr,
+                                   int* attempts = nullptr) {
     PrimOut r = cons_to_prim_attempt(c, d, tau, q, yT0, phi0, ux0, uy0, st);
     if (iters_total) *iters_total += st.iters;
+    if (first_iters) *first_iters = st.iters;
+    if (attempts) *attempts = 1;
     if (r.ok) return r;
     if (st.reason == PRR_VACUUM) return r;
     if (!(isf(yT0) && isf(ux0) && isf(uy0))) return r;
     const double nan = std::numeric_limits<double>::quiet_NaN();
     r = cons_to_prim_attempt(c, d, tau, q, nan, nan, nan, nan, st);
     if (iters_total) *iters_total += st.iters;
+    if (attempts) *attempts = 2;
     return r;
 }
 

@@ -72,11 +72,16 @@ void primitive_pass(Ctx& c, std::vector<double>& U, double tau) {
             Cons cs{safe_div(Ui[L.iD], tau), Ui[L.iSx], Ui[L.iSy], Ui[L.iE]};
             Diss d = diss_at(L, Ui);
             PrimStat st;
-            int its = 0;
+            int its = 0, it1 = 0, natt = 0;
+            const bool warm0 = isf(w.x0_yT[i]) && isf(w.x0_ux[i]) && isf(w.x0_uy[i]);
             PrimOut r = cons_to_prim(cs, d, tau, q, w.x0_yT[i], w.x0_phi[i], w.x0_ux[i], w.x0_uy[i],
-                                     st, &its);
+                                     st, &its, &it1, &natt);
             nit += its;
-            if (!w.dbg_iters.empty()) w.dbg_iters[i] = (st.reason == PRR_VACUUM) ? -1 : its;
+            if (!w.dbg_iters.empty()) {
+                w.dbg_iters[i] = (st.reason == PRR_VACUUM) ? -1 : its;
+                // class: 1 warm start, 2 fresh-seed retry, 4 ok; first attempt's iterations above bit 8
+                w.dbg_class[i] = (warm0 ? 1 : 0) | (natt == 2 ? 2 : 0) | (r.ok ? 4 : 0) | (it1 << 8);
+            }
             double T = r.T, mu = r.mu, ux = r.ux, uy = r.uy, n = r.n, e = r.e, P = r.P;
             bool ok = r.ok;
             if (!ok) {
@@ -107,7 +112,15 @@ void primitive_pass(Ctx& c, std::vector<double>& U, double tau) {
     w.primfail = nfail; w.vacuum = nvac; w.newton_iters += nit;
     if (!w.dbg_iters.empty()) {
         if (w.dbg_hist.size() < 256) w.dbg_hist.assign(256, 0);
-        for (int v : w.dbg_iters) if (v >= 0) w.dbg_hist[v < 255 ? v : 255] += 1;
+        if (w.dbg_hist2.size() < 2 * 8 * 256) w.dbg_hist2.assign(2 * 8 * 256, 0);
+        for (size_t i = 0; i < w.dbg_iters.size(); ++i) {
+            const int v = w.dbg_iters[i];
+            if (v < 0) continue;
+            w.dbg_hist[v < 255 ? v : 255] += 1;
+            const int cl = w.dbg_class[i] & 7, v1 = w.dbg_class[i] >> 8;
+            w.dbg_hist2[cl * 256 + (v < 255 ? v : 255)] += 1;
+            w.dbg_hist2[8 * 256 + cl * 256 + (v1 < 255 ? v1 : 255)] += 1;
+        }
     }
 }
 

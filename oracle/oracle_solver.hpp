@@ -70,6 +70,8 @@ struct Work {
     std::vector<double> shear_res;
     std::vector<int> dbg_iters;      // Newton iterations of each cell in the last primitive pass
     std::vector<int64_t> dbg_hist;   // histogram of per-cell iterations over all passes
+    std::vector<int64_t> dbg_hist2;  // [total|first-attempt][class][iterations]
+    std::vector<int> dbg_class;      // per cell: warm start, retry, ok, first-attempt iterations
     int64_t primfail = 0, vacuum = 0, newton_iters = 0;
     double amax = 0.0;
 };
