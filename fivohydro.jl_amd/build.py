@@ -2,7 +2,8 @@
 
 Every HIP translation unit is compiled with the packed-FP32 target feature off
 (all arithmetic here is FP64; the flag keeps hipcc from emitting packed FP32
-instructions in any helper code). Usage: python build.py [--jobs N]
+instructions in any helper code) and with multiply-add fusion limited to single
+expressions (-ffp-contract=on). Usage: python build.py [--jobs N]
 """
 import os
 import subprocess
@@ -13,9 +14,12 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(CSRC, "libfivo2d.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-SOURCES = ["fivo_kernels.hip", "fivo_relax.hip", "fivo_api.hip"]
+SOURCES = ["fivo_kernels.hip", "fivo_batch_kernels.hip", "fivo_relax.hip", "fivo_api.hip", "fivo_batch.hip"]
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-fno-gpu-rdc",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         # multiply-adds fuse only within one source expression, so the same body
+         # rounds the same way in every kernel it is inlined into
+         "-ffp-contract=on",
          "-Wno-unused-result", "-I", os.path.join(HERE, "..", "include")]
 
 
@@ -28,7 +32,7 @@ def _compile(src):
     return obj, r.stderr
 
 
-def build(jobs=3, verbose=False):
+def build(jobs=5, verbose=False):
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         res = list(ex.map(_compile, SOURCES))
     objs = [o for o, _ in res]

@@ -1,0 +1,365 @@
+// =============================================================================
+// fivo_batch_kernels.hip -- the ensemble kernels: one launch runs one phase of
+// run_sim_2d!'s step (main2D.jl:371-561) for every event of an ensemble.
+//
+// A block finds its event through the block -> event table of its grid kind
+// and runs the same body as the one-event kernel (fivo_cells.h) on that event's
+// arrays. The per-event control state (EvCtl) says whether the event takes
+// part in a phase (running, stage passed, step accepted), which buffers are the
+// stage input and output, and the stage time; the control kernels at the end of
+// this file advance it on the device, so a whole step of every event is a fixed
+// sequence of launches that is captured once in a hipGraph and replayed.
+//
+// Event tables and control state are read through the constant address space:
+// they do not change while a kernel runs, and the loads become scalar loads.
+// =============================================================================
+#include "fivo_cells.h"
+
+namespace fivo {
+using namespace fd;
+
+typedef const __attribute__((address_space(4))) EvView* CEv;
+typedef const __attribute__((address_space(4))) EvCtl* CCtl;
+typedef const __attribute__((address_space(4))) int* CInt;
+
+// an event's Params: the shared model part with the event's grid
+__device__ __forceinline__ Params ev_params(const BatchRun& R, CEv E) {
+    Params p = R.p0;
+    p.Nx = E->p.Nx; p.Ny = E->p.Ny; p.Nxtot = E->p.Nxtot; p.Nytot = E->p.Nytot; p.Ntot = E->p.Ntot;
+    p.pitch = E->p.pitch; p.dx = E->p.dx; p.dy = E->p.dy; p.xmin = E->p.xmin; p.ymin = E->p.ymin;
+    return p;
+}
+__device__ __forceinline__ DevWork ev_work(CEv E) {
+    DevWork w;
+    w.yT = E->w.yT; w.phi = E->w.phi; w.mu = E->w.mu; w.alpha = E->w.alpha; w.ux = E->w.ux; w.uy = E->w.uy;
+    w.P = E->w.P; w.n = E->w.n; w.e = E->w.e; w.vx = E->w.vx; w.vy = E->w.vy;
+    w.ok = E->w.ok; w.bad = E->w.bad; w.force = E->w.force; w.ftmp = E->w.ftmp;
+    w.x0_yT = E->w.x0_yT; w.x0_phi = E->w.x0_phi; w.x0_ux = E->w.x0_ux; w.x0_uy = E->w.x0_uy;
+    w.ux_prev = E->w.ux_prev; w.uy_prev = E->w.uy_prev; w.alpha_prev = E->w.alpha_prev; w.T_prev = E->w.T_prev;
+    w.nux_prev = E->w.nux_prev; w.nuy_prev = E->w.nuy_prev; w.shear_res = E->w.shear_res;
+    w.gradAx = E->w.gradAx; w.gradAy = E->w.gradAy;
+    return w;
+}
+
+__device__ __forceinline__ int block_event(const BatchRun& R, int kind) {
+    return __builtin_amdgcn_readfirstlane(((CInt)R.map[kind])[blockIdx.x]);
+}
+__device__ __forceinline__ bool ev_active(CCtl c, int phase) {
+    if (phase == PH_REFRESH) return c->accepted != 0;
+    return c->state == EV_RUNNING && c->stage == (phase == PH_INIT ? 0 : phase - 1);
+}
+// stage input role (offset from the accepted buffer) and the time of its primitive pass
+__device__ __forceinline__ int phase_in_role(int phase) {
+    return phase == PH_STAGE2 ? 1 : (phase == PH_STAGE3 ? 2 : 0);
+}
+__device__ __forceinline__ double phase_tau(CCtl c, int phase) {
+    return phase == PH_STAGE2 ? c->tau + c->D : (phase == PH_STAGE3 ? c->tau + 0.5 * c->D : c->tau);
+}
+
+// ---- ghost fill of buffer `role` -------------------------------------------------------
+__global__ __launch_bounds__(256) void kb_bc(BatchRun R, int phase, int role) {
+    const int e = block_event(R, KB_GHOST);
+    CCtl c = (CCtl)(R.ctl + e);
+    if (!ev_active(c, phase)) return;
+    CEv E = (CEv)(R.ev + e);
+    const Params p = ev_params(R, E);
+    const int t = (blockIdx.x - E->b0[KB_GHOST]) * 256 + threadIdx.x;
+    if (t < bc_nghost(p)) bc_cell(E->U[(c->cur + role) & 3], p, R.bc == FIVO_BC_PERIODIC, t);
+}
+
+// ---- primitive pass: warm part, one lane per cell --------------------------------------
+template <int EK, bool NCH>
+__global__ __launch_bounds__(256) void kb_prim(BatchRun R, int phase) {
+    const int e = block_event(R, KB_CELLS);
+    CCtl c = (CCtl)(R.ctl + e);
+    if (!ev_active(c, phase)) return;
+    CEv E = (CEv)(R.ev + e);
+    const Params p = ev_params(R, E);
+    const DevWork w = ev_work(E);
+    const double* U = E->U[(c->cur + phase_in_role(phase)) & 3];
+    const double tau = phase_tau(c, phase);
+    const int i = (blockIdx.x - E->b0[KB_CELLS]) * 256 + threadIdx.x;
+    unsigned long long its = 0, nfail = 0, nvac = 0;
+    if (i < p.Ntot) prim_cell<EK, NCH>(U, p, tau, w, i, e, &E->sc->ncold, E->jobs, its, nfail, nvac);
+    prim_counts(E->sc, its, nfail, nvac);
+}
+
+// ---- primitive pass: the event's queued cells, GW lanes per cell ------------------------
+template <int EK, bool NCH>
+__global__ __launch_bounds__(64) void kb_coop(BatchRun R, int phase) {
+    const int e = block_event(R, KB_COOP);
+    CCtl c = (CCtl)(R.ctl + e);
+    if (!ev_active(c, phase)) return;
+    CEv E = (CEv)(R.ev + e);
+    const int n = (int)E->sc->ncold;
+    const int lb = blockIdx.x - E->b0[KB_COOP];
+    const int ngroups = E->nb[KB_COOP] * (64 / GW);
+    int t = (lb * 64 + (int)threadIdx.x) / GW;
+    if (__builtin_amdgcn_readfirstlane(lb * (64 / GW)) >= n) return;
+    const Params p = ev_params(R, E);
+    const DevWork w = ev_work(E);
+    const double* U = E->U[(c->cur + phase_in_role(phase)) & 3];
+    const double tau = phase_tau(c, phase);
+    const PrimJob* jobs = E->jobs;
+    unsigned long long its = 0, nfail = 0, nvac = 0;
+    for (; t < n; t += ngroups) {
+        int itsum;
+        coop_job<EK, NCH>(U, p, tau, w, jobs[t], itsum, nfail, nvac);
+        if (glane() == 0) its += itsum;
+    }
+    prim_counts(E->sc, its, nfail, nvac);
+}
+
+// ---- the fused face / stage-update kernel --------------------------------------------
+// PH_INIT: the RHS at tau0 (its amax gives the first dt; output to the scratch
+// buffer); stage s of SSPRK3 (step_ssprk3_2d!, src2d/timestepper2d.jl:148-188):
+//   1: U1 = U + D k(U, tau)            2: U2 = 3/4 U + 1/4 (U1 + D k(U1, tau + D))
+//   3: U  = 1/3 U + 2/3 (U2 + D k(U2, tau + D/2)), each sanitised at tau + D
+template <int EK, bool NCH, int TX, int TY>
+__global__ __launch_bounds__(TX* TY) void kb_faces(BatchRun R, int phase) {
+    extern __shared__ double sF[];
+    const int e = block_event(R, KB_FACES);
+    CCtl c = (CCtl)(R.ctl + e);
+    if (!ev_active(c, phase)) return;
+    CEv E = (CEv)(R.ev + e);
+    FaceK K;
+    K.p = ev_params(R, E);
+    K.w = ev_work(E);
+    K.sc = E->sc;
+    K.tiles_y = E->tiles_y;
+    K.nmain = E->nmain;
+    K.remap = 0;
+    const int cur = c->cur;
+    const int s = phase == PH_INIT ? 1 : phase;
+    FaceLaunch& f = K.f;
+    f.Ub = E->U[(cur + s - 1) & 3];
+    f.Uold = E->U[cur & 3];
+    f.Uout = E->U[(cur + s) & 3];
+    f.dUout = nullptr;
+    f.mode = 1;
+    f.combine = s - 1;
+    f.tau = phase_tau(c, phase);
+    f.tau_state = c->tau + c->D;
+    f.D = c->D;
+    f.wold = s == 2 ? 0.75 : (s == 1 ? 1.0 : 0.0);
+    f.wnew = s == 2 ? 0.25 : 0.0;
+    f.force = nullptr;
+    faces_body<EK, NCH, TX, TY, 1>(K, blockIdx.x - E->b0[KB_FACES], sF);
+}
+
+// ---- near-bound cells of a stage output: cold inversion ---------------------------------
+template <int EK, bool NCH>
+__global__ __launch_bounds__(256) void kb_pending(BatchRun R, int phase) {
+    const int e = block_event(R, KB_INTERIOR);
+    CCtl c = (CCtl)(R.ctl + e);
+    if (!ev_active(c, phase)) return;
+    CEv E = (CEv)(R.ev + e);
+    if (E->sc->npending == 0) return;
+    const Params p = ev_params(R, E);
+    const int s = phase == PH_INIT ? 1 : phase;
+    pending_cell<EK, NCH>(E->U[(c->cur + s) & 3], p, c->tau + c->D, (uint8_t*)E->w.bad, E->sc,
+                          (blockIdx.x - E->b0[KB_INTERIOR]) * 256 + threadIdx.x);
+}
+
+// ---- history snapshot before a step (main2D.jl:458-463) ----------------------------------
+__global__ __launch_bounds__(256) void kb_history(BatchRun R) {
+    const int e = block_event(R, KB_CELLS);
+    CCtl c = (CCtl)(R.ctl + e);
+    if (!(c->state == EV_RUNNING && c->dohist)) return;
+    CEv E = (CEv)(R.ev + e);
+    const int t = (blockIdx.x - E->b0[KB_CELLS]) * 256 + threadIdx.x;
+    if (t < E->p.Ntot) history_cell(ev_work(E), t);
+}
+
+// ---- project_shear! per event, block partials ----------------------------------------
+__global__ __launch_bounds__(256) void kb_project_shear(BatchRun R, int correct) {
+    const int e = block_event(R, KB_INTERIOR);
+    CCtl c = (CCtl)(R.ctl + e);
+    if (!c->accepted) return;
+    CEv E = (CEv)(R.ev + e);
+    const Params p = ev_params(R, E);
+    const int lb = blockIdx.x - E->b0[KB_INTERIOR];
+    project_shear_block(E->U[c->cur & 3], p, ev_work(E), correct, lb * 256 + threadIdx.x, E->partials + 2 * lb);
+}
+
+// ---- control: one thread per event -------------------------------------------------------
+// what: 2 run start, 0 after the initial RHS, 1 step begin, 10+s after stage s,
+// 20 after relaxation, 21 step end
+__device__ __forceinline__ void reset_pass(DevScalars* sc) {
+    sc->amax_bits = 0; sc->nbad = 0; sc->nprimfail = 0; sc->nvacuum = 0; sc->npending = 0; sc->ncold = 0;
+}
+__global__ __launch_bounds__(64) void kb_ctl(BatchRun R, int what) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= R.nev) return;
+    EvCtl& c = R.ctl[e];
+    const EvView& E = R.ev[e];
+    DevScalars* sc = E.sc;
+    if (what == 0) {
+        // initial RHS done: its amax sets the first dt (run_sim_2d!, main2D.jl:384-400)
+        c.amax = __longlong_as_double((long long)sc->amax_bits);
+        c.nrhs += 1;
+        c.stage = 0;
+        reset_pass(sc);
+        return;
+    }
+    if (what == 2) {   // run start: the initial RHS follows
+        c.stage = 0;
+        reset_pass(sc);
+        return;
+    }
+    if (what == 1) {
+        c.accepted = 0;
+        c.dohist = 0;
+        if (c.state != EV_RUNNING) return;
+        if (c.fresh) {
+            if (!(c.tau < c.taufinal && c.nsteps < c.max_steps)) { c.state = EV_DONE; return; }
+            // compute_dt_2d (src2d/timestepper2d.jl:37-47)
+            const double amax = c.amax > 1e-6 ? c.amax : 1e-6;
+            const double dcfl = R.cfl * (E.p.dx < E.p.dy ? E.p.dx : E.p.dy) / amax;
+            const double dtau = R.cfltau * c.tau;
+            double D = dcfl < dtau ? dcfl : dtau;
+            const double rem = c.taufinal - c.tau;
+            D = D < rem ? D : rem;
+            if (D <= 0) { c.state = EV_DONE; return; }
+            c.D = D;
+            c.nhalve = 0;
+            if (c.have_hist || c.nsteps > 0) { c.dohist = 1; c.have_hist = 1; }
+        }
+        c.stage = 0;
+        reset_pass(sc);
+        return;
+    }
+    if (what >= 11 && what <= 13) {
+        const int s = what - 10;
+        if (!(c.state == EV_RUNNING && c.stage == s - 1)) return;
+        c.nrhs += 1;
+        if (sc->nbad == 0) {
+            c.stage = s;
+            if (s == 3) c.amax = __longlong_as_double((long long)sc->amax_bits);
+            // without relaxation the step's primitive counts are those of its last pass
+            if (s == 3 && !R.dissip) { c.nprimfail += (long long)sc->nprimfail; c.nvacuum += (long long)sc->nvacuum; }
+        } else {
+            c.stage = -1;   // attempt failed: no later stage of this replay runs
+        }
+        if (s == 3 || c.stage < 0) {
+            // step_ssprk3_2d! outcome: accept, or halve dt and retry next replay
+            if (c.stage == 3) {
+                c.cur = (c.cur + 3) & 3;
+                c.tau += c.D;
+                c.nsteps += 1;
+                c.accepted = 1;
+                c.fresh = 1;
+            } else {
+                c.nhalvings += 1;
+                c.nhalve += 1;
+                c.D *= 0.5;
+                c.fresh = 0;
+                if (c.nhalve > R.max_halve) c.state = EV_FAILED;
+            }
+        }
+        reset_pass(sc);
+        return;
+    }
+    if (what == 20) {   // relaxation wrote U[(cur+1)%4]
+        if (c.accepted && R.dissip) {
+            c.cur = (c.cur + 1) & 3;
+            c.nprimfail += (long long)sc->nprimfail;
+            c.nvacuum += (long long)sc->nvacuum;
+        }
+        return;
+    }
+    if (what == 21) {   // step end: shear residual of the accepted step
+        if (c.accepted && E.p.enable_shear) {
+            const int nbi = E.nb[KB_INTERIOR];
+            double m = 0.0;
+            for (int b = 0; b < nbi; ++b) m = E.partials[2 * b] > m ? E.partials[2 * b] : m;
+            c.max_shear = m > c.max_shear ? m : c.max_shear;
+        }
+        if (c.state == EV_RUNNING && !(c.tau < c.taufinal && c.nsteps < c.max_steps) && c.fresh) c.state = EV_DONE;
+        if (c.state == EV_RUNNING) atomicAdd(&R.bs->nrunning, 1ull);
+        return;
+    }
+}
+__global__ void kb_zero_running(BatchRun R) { R.bs->nrunning = 0; }
+
+// ---- launchers -------------------------------------------------------------------------------
+#define EOS_DISPATCH_B(b, BODY)                                                      \
+    do {                                                                             \
+        if ((b).p0.eos.kind == 1) { constexpr int EK = 1; constexpr bool NCH = true; BODY; } \
+        else if ((b).p0.eos.has_charge) { constexpr int EK = 2; constexpr bool NCH = true; BODY; } \
+        else { constexpr int EK = 0; constexpr bool NCH = false; BODY; }             \
+    } while (0)
+
+void batch_tile_shape(int tile, int* tx, int* ty) {
+    static const int T[BATCH_NTILES][2] = {{4, 16}, {8, 16}, {8, 32}, {16, 16}};
+    *tx = T[tile][0]; *ty = T[tile][1];
+}
+int batch_face_blocks(const Params& p, int tile, int* tiles_y, int* nmain) {
+    int nb = 0;
+    switch (tile) {
+        case 0: face_grid<4, 16>(p, tiles_y, nmain, &nb); break;
+        case 1: face_grid<8, 16>(p, tiles_y, nmain, &nb); break;
+        case 2: face_grid<8, 32>(p, tiles_y, nmain, &nb); break;
+        default: face_grid<16, 16>(p, tiles_y, nmain, &nb); break;
+    }
+    return nb;
+}
+
+hipError_t launch_batch_bc(Batch& b, int phase, int role) {
+    hipLaunchKernelGGL(kb_bc, dim3(b.nblocks[KB_GHOST]), dim3(256), 0, b.stream, b.run, phase, role);
+    return hipGetLastError();
+}
+hipError_t launch_batch_prim(Batch& b, int phase) {
+    EOS_DISPATCH_B(b, hipLaunchKernelGGL((kb_prim<EK, NCH>), dim3(b.nblocks[KB_CELLS]), dim3(256), 0, b.stream,
+                                         b.run, phase));
+    EOS_DISPATCH_B(b, hipLaunchKernelGGL((kb_coop<EK, NCH>), dim3(b.nblocks[KB_COOP]), dim3(64), 0, b.stream,
+                                         b.run, phase));
+    return hipGetLastError();
+}
+template <int EK, bool NCH, int TX, int TY>
+static hipError_t batch_faces_tx(Batch& b, int phase) {
+    constexpr int NF = (TX + 1) * TY + TX * (TY + 1);
+    const int nvf = b.p0.iPQ0 >= 0 ? b.p0.iPQ0 : b.p0.nv;
+    const size_t lds = sizeof(double) * (size_t)nvf * NF;
+    static size_t lds_set = 0;   // the attribute is set once, before any graph capture
+    if (lds > 65536 && lds > lds_set) {
+        hipError_t e = hipFuncSetAttribute((const void*)kb_faces<EK, NCH, TX, TY>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+        lds_set = lds;
+    }
+    hipLaunchKernelGGL((kb_faces<EK, NCH, TX, TY>), dim3(b.nblocks[KB_FACES]), dim3(TX * TY), lds, b.stream, b.run,
+                       phase);
+    return hipGetLastError();
+}
+hipError_t launch_batch_faces(Batch& b, int phase, int tile) {
+    hipError_t e = hipSuccess;
+    switch (tile) {
+        case 0: EOS_DISPATCH_B(b, e = (batch_faces_tx<EK, NCH, 4, 16>(b, phase))); break;
+        case 1: EOS_DISPATCH_B(b, e = (batch_faces_tx<EK, NCH, 8, 16>(b, phase))); break;
+        case 2: EOS_DISPATCH_B(b, e = (batch_faces_tx<EK, NCH, 8, 32>(b, phase))); break;
+        default: EOS_DISPATCH_B(b, e = (batch_faces_tx<EK, NCH, 16, 16>(b, phase))); break;
+    }
+    return e;
+}
+hipError_t launch_batch_pending(Batch& b, int phase) {
+    EOS_DISPATCH_B(b, hipLaunchKernelGGL((kb_pending<EK, NCH>), dim3(b.nblocks[KB_INTERIOR]), dim3(256), 0,
+                                         b.stream, b.run, phase));
+    return hipGetLastError();
+}
+hipError_t launch_batch_history(Batch& b) {
+    hipLaunchKernelGGL(kb_history, dim3(b.nblocks[KB_CELLS]), dim3(256), 0, b.stream, b.run);
+    return hipGetLastError();
+}
+hipError_t launch_batch_project_shear(Batch& b) {
+    hipLaunchKernelGGL(kb_project_shear, dim3(b.nblocks[KB_INTERIOR]), dim3(256), 0, b.stream, b.run,
+                       b.shear_project);
+    return hipGetLastError();
+}
+hipError_t launch_batch_ctl(Batch& b, int what) {
+    if (what == -1) hipLaunchKernelGGL(kb_zero_running, dim3(1), dim3(1), 0, b.stream, b.run);
+    else hipLaunchKernelGGL(kb_ctl, dim3((b.nev + 63) / 64), dim3(64), 0, b.stream, b.run, what);
+    return hipGetLastError();
+}
+
+}  // namespace fivo

@@ -55,8 +55,9 @@ struct DevScalars {
 // Per-cell work arrays on the device (Work2D, src2d/work2d.jl:15-100)
 // a cell queued by the warm primitive kernel: resume its Newton at iteration
 // `it` from the iterate kept in the warm-start arrays, or (it = 0) solve it
-// from the cold seed (it >= 1000: only the staged fallback is left)
-struct PrimJob { int cell, it; };
+// from the cold seed (it >= 1000: only the staged fallback is left); ev is the
+// event of an ensemble (0 for one event)
+struct PrimJob { int cell, it, ev; };
 
 struct DevWork {
     double *yT, *phi, *mu, *alpha, *ux, *uy, *P, *n, *e, *vx, *vy;
@@ -129,5 +130,96 @@ hipError_t launch_reduce_diag(Ctx& c, const double* U, int what);
 hipError_t launch_project_shear(Ctx& c, double* U, int correct);
 hipError_t launch_relax(Ctx& c, double* U, double tau, double dtau);
 hipError_t launch_layout(Ctx& c, const double* aos_dev, double* soa, int to_soa);
+
+// ---- ensembles (fivo_batch.hip, fivo_batch_kernels.hip) -----------------------
+// An ensemble is a set of events, each a one-event context (its own grid, state
+// buffers, work arrays and scalars) sharing one model. The ensemble kernels run
+// one phase of the time step for every event in one launch: each grid kind maps
+// a block to (event, local block) through a block -> event table.
+enum { KB_CELLS = 0,      // 256 cells of Ntot per block
+       KB_INTERIOR = 1,   // 256 interior cells per block
+       KB_GHOST = 2,      // 256 ghost cells per block
+       KB_FACES = 3,      // face tiles + ghost-face blocks
+       KB_COOP = 4,       // 64-lane blocks of the cooperative primitive kernel
+       KB_N = 5 };
+// per event, device resident, fixed during a run
+struct EvView {
+    Params p;
+    DevWork w;
+    double* U[4];
+    DevScalars* sc;
+    double* partials;     // 2 per interior block
+    PrimJob* jobs;        // queued cells of the event's primitive pass (count: sc->ncold)
+    int tiles_y, nmain;   // face tiling of the ensemble's tile shape
+    int b0[KB_N];         // first block of the event in each grid kind
+    int nb[KB_N];         // blocks of the event in each grid kind
+};
+// per event run control: the state of run_sim_2d!'s loop (main2D.jl:371-561)
+enum { EV_RUNNING = 0, EV_DONE = 1, EV_FAILED = 2 };
+struct EvCtl {
+    double tau, D, taufinal, amax, max_shear;
+    int cur;        // index of the accepted state among U[0..3]
+    int state;      // EV_*
+    int fresh;      // next attempt starts a new step (history snapshot, new dt)
+    int stage;      // stages of the current attempt passed so far
+    int nhalve;     // halvings of the current step
+    int have_hist;
+    int accepted;   // the current replay accepted a step
+    int dohist;
+    long long nsteps, max_steps, nprimfail, nvacuum, nhalvings, nrhs;
+};
+struct BatchScalars {
+    unsigned long long nrunning;   // events still running after the last replay
+    unsigned long long pad[7];
+};
+struct BatchRun {                  // kernel arguments shared by the ensemble kernels
+    Params p0;                     // the model part of every event's Params
+    const EvView* ev;
+    EvCtl* ctl;
+    BatchScalars* bs;
+    const int* map[KB_N];
+    int nev;
+    int bc;
+    int max_halve;
+    int dissip;
+    double cfl, cfltau;
+};
+// phases of one replay of the step graph
+enum { PH_INIT = 0, PH_STAGE1 = 1, PH_STAGE2 = 2, PH_STAGE3 = 3, PH_REFRESH = 4 };
+// host side of an ensemble
+struct Batch {
+    std::vector<Ctx*> evs;          // one-event contexts: state, work arrays, scalars
+    int nev = 0;
+    Params p0;                      // model part shared by the events (grid of event 0)
+    int dev = 0;
+    hipStream_t stream = nullptr;
+    EvView* d_ev = nullptr;
+    EvCtl* d_ctl = nullptr;
+    EvCtl* h_ctl = nullptr;         // pinned mirror
+    BatchScalars* bs = nullptr;
+    BatchScalars* bs_host = nullptr;
+    int* d_map[KB_N] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    int nblocks[KB_N] = {0, 0, 0, 0, 0};
+    std::vector<EvView> h_ev;
+    int tile = -1;                  // face tile index chosen by the autotuner
+    float tile_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int graph_tile = -1, graph_dissip = -1;
+    int shear_project = 0;          // project_shear! corrects pi^eta_eta
+    BatchRun run{};
+    std::string err;
+};
+hipError_t launch_batch_bc(Batch& b, int phase, int role);
+hipError_t launch_batch_prim(Batch& b, int phase);
+hipError_t launch_batch_faces(Batch& b, int phase, int tile);
+hipError_t launch_batch_pending(Batch& b, int phase);
+hipError_t launch_batch_history(Batch& b);
+hipError_t launch_batch_relax(Batch& b);
+hipError_t launch_batch_project_shear(Batch& b);
+hipError_t launch_batch_ctl(Batch& b, int what);
+int batch_face_blocks(const Params& p, int tile, int* tiles_y, int* nmain);
+constexpr int BATCH_NTILES = 4;    // face tile candidates of the autotuner
+void batch_tile_shape(int tile, int* tx, int* ty);
 
 }  // namespace fivo

@@ -1,153 +1,17 @@
 // =============================================================================
-// fivo_kernels.hip -- the FP64 HIP kernels of the 2+1D RHS and time step (gfx950).
-//
-//   k_bc        ghost fill, outflow or periodic             src2d/bc2d.jl:14-65
-//   k_prim      primitive pass (Newton per cell)            src2d/rhs2d.jl:67-160
-//   k_faces     fused: MUSCL + forward map + HLLE for every face of a tile into
-//               LDS, then per cell the flux divergence + Bjorken source, the RK
-//               combination, the stage sanitiser and the admissibility test
-//                 src2d/reconstruction2d.jl:27-103, rhs2d.jl:169-371,
-//                 fluxes2d.jl:24-177, timestepper2d.jl:113-188, floors2d.jl:24-337
-//   k_expand    MOOD mask dilation                          src2d/floors2d.jl:369-383
-//   + small kernels: standalone sanitiser / admissibility, IC forward map,
-//     history snapshot, diagnostics reductions, shear projection, layout copies.
+// fivo_kernels.hip -- the FP64 HIP kernels of one event's 2+1D RHS and time step
+// (gfx950): thin wrappers around the bodies of fivo_cells.h, plus the kernels
+// only the one-event API needs (MOOD dilation, standalone sanitiser and
+// admissibility, IC forward map, layout copies, diagnostics) and the launchers.
 // =============================================================================
-#include <algorithm>
-#include "fivo_ctx.h"
+#include "fivo_cells.h"
 
 namespace fivo {
 using namespace fd;
 
-#define P_ (p.pitch)
-
-// ---- helpers ---------------------------------------------------------------------
-__device__ __forceinline__ Diss load_diss(const double* __restrict__ U, const Params& p, int i) {
-    Diss d;
-    d.nux = p.hasNu ? U[p.iNux * P_ + i] : 0.0;
-    d.nuy = p.hasNu ? U[p.iNuy * P_ + i] : 0.0;
-    d.Pi = p.hasPi ? U[p.iPi * P_ + i] : 0.0;
-    d.pixx = p.hasShear ? U[p.iPixx * P_ + i] : 0.0;
-    d.pixy = p.hasShear ? U[p.iPixy * P_ + i] : 0.0;
-    d.piyy = p.hasShear ? U[p.iPiyy * P_ + i] : 0.0;
-    d.pieta = p.hasShear ? U[p.iPieta * P_ + i] : 0.0;
-    return d;
-}
-
-__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-__device__ __forceinline__ double wave_max_pos(double v) {   // NaN ignored, like `a > amax`
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) { double u = __shfl_xor(v, o, 64); v = u > v ? u : v; }
-    return v;
-}
-__device__ __forceinline__ void atomic_max_pos(unsigned long long* a, double v) {
-    if (v > 0.0) atomicMax(a, (unsigned long long)__double_as_longlong(v));
-}
-
-// XCD-aware block order: consecutive logical tiles land on the same XCD (and L2)
-__device__ __forceinline__ int xcd_remap(int b, int nb) {
-    const int nx = 8;
-    int xcd = b % nx, idx = b / nx, per = nb / nx, rem = nb % nx;
-    return xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
-}
-
-// ---- ghost fill (src2d/bc2d.jl:14-65) -----------------------------------------------
-// Outflow x-then-y copies equal a copy from the clamped cell; periodic x-then-y
-// wraps equal a copy from the doubly wrapped cell. One thread per ghost cell.
 __global__ __launch_bounds__(256) void k_bc(double* __restrict__ U, Params p, int periodic) {
-    const int ng = p.ng;
-    const int nA = ng * p.Nytot, nB = ng * p.Nx;
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    int ix, iy;
-    if (t < nA) { ix = t / p.Nytot; iy = t - ix * p.Nytot; }
-    else if ((t -= nA) < nA) { ix = ng + p.Nx + t / p.Nytot; iy = t % p.Nytot; }
-    else if ((t -= nA) < nB) { ix = ng + t / ng; iy = t % ng; }
-    else if ((t -= nB) < nB) { ix = ng + t / ng; iy = ng + p.Ny + t % ng; }
-    else return;
-    int sx, sy;
-    if (periodic) {
-        sx = ix < ng ? ix + p.Nx : (ix >= ng + p.Nx ? ix - p.Nx : ix);
-        sy = iy < ng ? iy + p.Ny : (iy >= ng + p.Ny ? iy - p.Ny : iy);
-    } else {
-        sx = ix < ng ? ng : (ix >= ng + p.Nx ? ng + p.Nx - 1 : ix);
-        sy = iy < ng ? ng : (iy >= ng + p.Ny ? ng + p.Ny - 1 : iy);
-    }
-    const int dst = ix * p.Nytot + iy, src = sx * p.Nytot + sy;
-    for (int a = 0; a < p.nv; ++a) U[a * P_ + dst] = U[a * P_ + src];
-}
-
-// ---- primitive pass (src2d/rhs2d.jl:67-160) ---------------------------------------
-// Two kernels. k_prim gives every cell whose Newton starts warm (the production
-// path) or is vacuum PRIM_WARM_ITCAP iterations, one lane per cell; almost all
-// cells converge there. A cell still iterating is paused (its iterate kept in
-// the warm-start arrays) and queued; a cell with no warm start, or whose warm
-// solve failed (the fresh-seed retry of cons_to_prim_2d!,
-// src2d/primrec2d.jl:465-486), is queued for the cold seed. k_prim_coop then
-// solves the queue with GW lanes per cell: the slow cells set the pass's
-// latency, and cooperating lanes cut each iteration to about two residual
-// evaluations on the critical path instead of nine.
-constexpr int PRIM_WARM_ITCAP = 6;
-constexpr int PRIM_JOB_TAIL = 1000;   // job.it = PRIM_JOB_TAIL + iters: only the staged fallback is left
-__device__ __forceinline__ void prim_finalize(const Params& p, const DevWork& w, int i, const PrimRes& r0,
-                                              const Diss& d, unsigned long long& nfail, unsigned long long& nvac) {
-    double T = r0.T, mu = r0.mu, ux = r0.ux, uy = r0.uy, n = r0.n, e = r0.e, P = r0.P;
-    if (!r0.ok) {
-        if (r0.reason == PRR_VACUUM) nvac += 1; else nfail += 1;
-        T = T_MIN; mu = 0.0; ux = 0.0; uy = 0.0;
-        // eos_Pne_2d(T_MIN, 0): the light sector at T_MIN, no charm density
-        if (p.eos.kind == 1) {
-            double Pl, dP, d2P;
-            light_eval<1>(T_MIN, p.eos, &Pl, &dP, &d2P);
-            P = Pl + 0.0; e = jmax((-Pl + T_MIN * dP) + 0.0, 0.0);
-        } else {
-            // conformal: the charm sector carries exp(-m/T_MIN) = 0
-            const double Pl = p.eos.aSB * pow4(T_MIN) * p.eos.fmGeV3, el = 3.0 * Pl;
-            double Ph = 0.0, eh = 0.0;
-            if (p.eos.conf_charm) {
-                const double x = p.eos.m / T_MIN;
-                const double K2 = besselkx_asym(2, x), K1 = besselkx_asym(1, x);
-                const double Ef = exp((0.0 - p.eos.m) / T_MIN);
-                Ph = p.eos.Aconf * (T_MIN * T_MIN) * Ef * K2;
-                eh = p.eos.Aconf * (T_MIN * T_MIN) * Ef * (3.0 * K2 + x * K1);
-                if (p.eos.hq_fm) { Ph *= p.eos.fmGeV3; eh *= p.eos.fmGeV3; }
-            }
-            P = Pl + Ph; e = jmax(el + eh, 0.0);
-        }
-        n = 0.0;
-    }
-    const double Tm = jmax(T, T_MIN);
-    const double yT = log(Tm);
-    const double phi = jclamp((mu - p.eos.m) / Tm, -PHI_CAP, PHI_CAP);
-    w.yT[i] = yT; w.phi[i] = phi; w.mu[i] = mu; w.alpha[i] = mu / Tm;
-    w.ux[i] = ux; w.uy[i] = uy; w.P[i] = P; w.n[i] = n; w.e[i] = e;
-    w.ok[i] = r0.ok ? 1 : 0;
-    const double ut = sqrt(1 + ux * ux + uy * uy), iut = safe_inv(ut);
-    w.vx[i] = ux * iut; w.vy[i] = uy * iut;
-    const double nan = __longlong_as_double(0x7ff8000000000000LL);
-    w.x0_yT[i] = r0.ok ? yT : nan;
-    w.x0_phi[i] = r0.ok ? phi : nan;
-    w.x0_ux[i] = r0.ok ? ux : nan;
-    w.x0_uy[i] = r0.ok ? uy : nan;
-    if (p.hasShear) {
-        double tt, tx, ty;
-        shear_tau_row(ux, uy, ut, d.pixx, d.pixy, d.piyy, &tt, &tx, &ty);
-        double R = -tt + d.pixx + d.piyy + d.pieta;
-        double scale = jmax(jmax(jmax(jmax(fabs(tt), fabs(d.pixx)), fabs(d.piyy)), fabs(d.pieta)), fabs(d.pixy));
-        w.shear_res[i] = scale > TINY ? fabs(R) / scale : 0.0;
-    }
-}
-
-__device__ __forceinline__ void prim_counts(DevScalars* sc, unsigned long long its, unsigned long long nfail,
-                                            unsigned long long nvac) {
-    its = wave_sum(its); nfail = wave_sum(nfail); nvac = wave_sum(nvac);
-    if ((threadIdx.x & 63) == 0) {
-        if (its) atomicAdd(&sc->newton_iters, its);
-        if (nfail) atomicAdd(&sc->nprimfail, nfail);
-        if (nvac) atomicAdd(&sc->nvacuum, nvac);
-    }
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < bc_nghost(p)) bc_cell(U, p, periodic, t);
 }
 
 template <int EK, bool NCH>
@@ -155,43 +19,7 @@ __global__ __launch_bounds__(256) void k_prim(const double* __restrict__ U, Para
                                               DevScalars* sc, PrimJob* jobs) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long its = 0, nfail = 0, nvac = 0;
-    if (i < p.Ntot) {
-        const double D = safe_div(U[i], tau), Sx = U[P_ + i], Sy = U[2 * P_ + i], E = U[3 * P_ + i];
-        const CellIn c = make_cell<NCH>(D, Sx, Sy, E, load_diss(U, p, i));
-        const double yT0 = w.x0_yT[i], ux0 = w.x0_ux[i], uy0 = w.x0_uy[i];
-        const bool vac = !isf(E) || E <= E_VAC || !isf(D) || D < 0.0;
-        const bool warm = isf(yT0) && isf(ux0) && isf(uy0);
-        int job = 0;                                 // < 0: done here; 0: cold seed; > 0: resume
-        if (vac || warm) {
-            NewtonState s;
-            PrimRes r;
-            bool fin = newton_begin<EK, NCH, false, false>(c, yT0, w.x0_phi[i], ux0, uy0, p.eos, s, r);
-            bool paused = false;
-            if (!fin) {
-                newton_iterate<EK, NCH, false>(s, c, p.eos, PRIM_WARM_ITCAP);
-                paused = newton_paused(s);
-                if (paused) {
-                    job = s.it;
-                    w.x0_yT[i] = s.x[0]; w.x0_phi[i] = s.x[1]; w.x0_ux[i] = s.x[2]; w.x0_uy[i] = s.x[3];
-                } else {
-                    r = newton_finish<EK, NCH, false, false>(s, c, p.eos);
-                    if (r.reason == PRR_UNSET) {      // staged phi fallback: done by the group
-                        paused = true;
-                        job = PRIM_JOB_TAIL + s.iters;
-                        w.x0_yT[i] = s.x[0]; w.x0_phi[i] = s.x[1]; w.x0_ux[i] = s.x[2]; w.x0_uy[i] = s.x[3];
-                    }
-                }
-            }
-            if (!paused) {
-                its = r.iters;
-                if (r.ok || r.reason == PRR_VACUUM) { prim_finalize(p, w, i, r, c.d, nfail, nvac); job = -1; }
-            }
-        }
-        if (job >= 0) {
-            const unsigned long long k = atomicAdd(&sc->ncold, 1ull);
-            jobs[k] = PrimJob{i, job};
-        }
-    }
+    if (i < p.Ntot) prim_cell<EK, NCH>(U, p, tau, w, i, 0, &sc->ncold, jobs, its, nfail, nvac);
     prim_counts(sc, its, nfail, nvac);
 }
 
@@ -200,446 +28,34 @@ __global__ __launch_bounds__(256) void k_prim_coop(const double* __restrict__ U,
                                                    DevScalars* sc, const PrimJob* __restrict__ jobs) {
     const int n = (int)sc->ncold;
     const int ngroups = gridDim.x * (blockDim.x / GW);
-    const bool lead = glane() == 0;
     unsigned long long its = 0, nfail = 0, nvac = 0;
     for (int t = (blockIdx.x * blockDim.x + threadIdx.x) / GW; t < n; t += ngroups) {
-        const PrimJob jb = jobs[t];
-        const int i = jb.cell;
-        const double D = safe_div(U[i], tau), Sx = U[P_ + i], Sy = U[2 * P_ + i], E = U[3 * P_ + i];
-        const CellIn c = make_cell<NCH>(D, Sx, Sy, E, load_diss(U, p, i));
-        NewtonState s;
-        PrimRes r;
-        int itsum = 0;
-        bool resume = jb.it > 0;                      // else: solve from the fresh seed
-        const double nan = __longlong_as_double(0x7ff8000000000000LL);
-#pragma unroll 1
-        for (int pass = 0; pass < 2; ++pass) {        // a failed resumed solve retries from the seed
-            bool fin = false;
-            if (resume) {
-                s.x[0] = w.x0_yT[i]; s.x[1] = w.x0_phi[i]; s.x[2] = w.x0_ux[i]; s.x[3] = w.x0_uy[i];
-                s.it = jb.it; s.iters = 0; s.reason = PRR_UNSET; s.converged = false;
-                newton_set<EK, NCH>(s, c, p.eos);
-            } else {
-                fin = newton_begin<EK, NCH, true, true>(c, nan, nan, nan, nan, p.eos, s, r);
-            }
-            if (!fin) {
-                if (resume && jb.it >= PRIM_JOB_TAIL) s.iters = jb.it - PRIM_JOB_TAIL;   // tail only
-                else newton_iterate<EK, NCH, true>(s, c, p.eos, NEWTON_MAXIT);
-                r = newton_finish<EK, NCH, true>(s, c, p.eos);
-            }
-            itsum += r.iters;
-            if (!resume || r.ok || r.reason == PRR_VACUUM) break;
-            resume = false;
-        }
-        if (lead) {
-            its += itsum;
-            prim_finalize(p, w, i, r, c.d, nfail, nvac);
-        }
+        int itsum;
+        coop_job<EK, NCH>(U, p, tau, w, jobs[t], itsum, nfail, nvac);
+        if (glane() == 0) its += itsum;
     }
     prim_counts(sc, its, nfail, nvac);
 }
 
-// ---- face physics -------------------------------------------------------------------
-struct FPrim { double T, ux, uy, n, e, P, cs2; };
-
-// prim_to_cons_2d! (src2d/primrec2d.jl:753-796) for a face state; also returns
-// c_s^2 at the same T (eos_cs2, src/eos.jl:471-476) from the same light-sector exp.
-template <int EK>
-__device__ __forceinline__ bool p2c(double T, double mu, double ux, double uy, const Diss& d, double tau,
-                                    const EosP& q, double cons[4], FPrim& pr) {
-    const double Tc = jmax(T, T_MIN);
-    const double ut = sqrt(1 + ux * ux + uy * uy);
-    double P, n, e, cs2;
-    if (EK != 1) {
-        TPart t = eos_tpart<EK>(Tc, q);
-        eos_assemble<EK>(t, mu, q, &P, &n, &e);
-        cs2 = 1.0 / 3.0;
-    } else {
-        double Pl, dP, d2P;
-        light_eval<2>(Tc, q, &Pl, &dP, &d2P);
-        TPart t;
-        t.T = Tc; t.Pl = Pl; t.el = -Pl + Tc * dP; t.x = jmin(q.m / Tc, 100000.0);
-        t.K2 = (!isf(t.x) || t.x <= 0.0) ? 0.0 : besselk2x_fast(t.x);
-        t.K1 = 0.0;
-        eos_assemble<1>(t, mu, q, &P, &n, &e);
-        cs2 = (d2P == 0.0) ? (1.0 / 3.0) : (dP / (Tc * d2P));
-    }
-    if (!(isf(P) && isf(n) && isf(e)) || e <= 0.0) return false;
-    const double weff = e + P + d.Pi, Ptot = P + d.Pi;
-    double tt, tx, ty;
-    shear_tau_row(ux, uy, ut, d.pixx, d.pixy, d.piyy, &tt, &tx, &ty);
-    cons[0] = tau * (n * ut + nu_tau(ux, uy, ut, d.nux, d.nuy));
-    cons[1] = weff * ut * ux + tx;
-    cons[2] = weff * ut * uy + ty;
-    cons[3] = weff * ut * ut - Ptot + tt;
-    pr = {Tc, ux, uy, n, e, P, cs2};
-    return true;
-}
-
-// flux_cell_2d! (src2d/fluxes2d.jl:24-69), conserved rows (advect_* are off)
-__device__ __forceinline__ void flux4(const FPrim& s, const Diss& d, double tau, int dir, double F[4]) {
-    const double ux = s.ux, uy = s.uy;
-    const double ut = sqrt(1 + ux * ux + uy * uy);
-    const double Ptot = s.P + d.Pi, weff = (s.e + s.P) + d.Pi;
-    double tt, tx, ty;
-    shear_tau_row(ux, uy, ut, d.pixx, d.pixy, d.piyy, &tt, &tx, &ty);
-    if (dir == 0) {
-        F[0] = tau * (s.n * ux + d.nux);
-        F[1] = weff * ux * ux + Ptot + d.pixx;
-        F[2] = weff * ux * uy + d.pixy;
-        F[3] = weff * ut * ux + tx;
-    } else {
-        F[0] = tau * (s.n * uy + d.nuy);
-        F[1] = weff * ux * uy + d.pixy;
-        F[2] = weff * uy * uy + Ptot + d.piyy;
-        F[3] = weff * ut * uy + ty;
-    }
-}
-
-// wavespeeds_2d (src2d/fluxes2d.jl:87-114)
-__device__ __forceinline__ void wavespeeds(const FPrim& s, double Pi, double pinn, int dir, double* lm, double* lp) {
-    double cs2 = jclamp(s.cs2, 0.0, 0.999);
-    const double wv = s.e + s.P;
-    if (wv > TINY) cs2 = jmin(cs2 + (fabs(Pi) + fabs(pinn)) / wv, 0.999);
-    const double ut = sqrt(1 + s.ux * s.ux + s.uy * s.uy), iut = safe_inv(ut);
-    const double vx = s.ux * iut, vy = s.uy * iut;
-    const double v2 = vx * vx + vy * vy;
-    const double vn = dir == 0 ? vx : vy;
-    const double den = 1 - v2 * cs2;
-    if (den <= TINY) { *lm = -1.0; *lp = 1.0; return; }
-    const double cs = safe_sqrt(cs2);
-    const double rad = (1 - v2) * (den - vn * vn * (1 - cs2));
-    const double disc = cs * safe_sqrt(jmax(rad, 0.0));
-    *lm = jclamp((vn * (1 - cs2) - disc) / den, -1.0, 1.0);
-    *lp = jclamp((vn * (1 - cs2) + disc) / den, -1.0, 1.0);
-}
-
-struct FaceK {
-    Params p;
-    FaceLaunch f;
-    DevWork w;
-    DevScalars* sc;
-    int tiles_y, nmain;
-};
-
-// One face: left cell i, right cell i + st (st = Nytot for x, 1 for y);
-// k0 = 0-based index along dir of the left cell. _faces_2d! (src2d/rhs2d.jl:282-371).
-// Writes the nv-row numerical flux to out[a*ostride] when WRITE; returns max |s|.
-template <int EK, bool NCH, bool WRITE>
-__device__ __forceinline__ double face(const FaceK& K, int i, int dir, int k0, bool forced, double* out,
-                                       int ostride) {
-    const Params& p = K.p;
-    const DevWork& w = K.w;
-    const double* __restrict__ U = K.f.Ub;
-    const double tau = K.f.tau;
-    const int st = dir == 0 ? p.Nytot : 1;
-    const int kmax = dir == 0 ? p.Nxtot : p.Nytot;
-    const int ip = i + st;
-    const Diss dL = load_diss(U, p, i), dR = load_diss(U, p, ip);
-    const double yTi = w.yT[i], yTp = w.yT[ip], uxi = w.ux[i], uxp = w.ux[ip], uyi = w.uy[i], uyp = w.uy[ip];
-    double cL[4], cR[4];
-    FPrim sL, sR;
-    bool good = false, okL = false, okR = false;
-    if (!forced) {
-        const double phii = w.phi[i], phip = w.phi[ip];
-        double ul[4], ur[4];
-        if (k0 < p.ng || k0 >= kmax - p.ng - 1) {
-            ul[0] = yTi; ul[1] = phii; ul[2] = uxi; ul[3] = uyi;
-            ur[0] = yTp; ur[1] = phip; ur[2] = uxp; ur[3] = uyp;
-        } else {
-            const int im = i - st, ipp = ip + st;
-            const double fm[4] = {w.yT[im], w.phi[im], w.ux[im], w.uy[im]};
-            const double f0[4] = {yTi, phii, uxi, uyi};
-            const double f1[4] = {yTp, phip, uxp, uyp};
-            const double f2[4] = {w.yT[ipp], w.phi[ipp], w.ux[ipp], w.uy[ipp]};
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const double si = mc_limiter(f0[v] - fm[v], f1[v] - f0[v]);
-                const double sp = mc_limiter(f1[v] - f0[v], f2[v] - f1[v]);
-                ul[v] = jclamp(f0[v] + 0.5 * si, jmin(jmin(fm[v], f0[v]), f1[v]), jmax(jmax(fm[v], f0[v]), f1[v]));
-                ur[v] = jclamp(f1[v] - 0.5 * sp, jmin(jmin(f0[v], f1[v]), f2[v]), jmax(jmax(f0[v], f1[v]), f2[v]));
-            }
-        }
-        const double TL = exp(ul[0]), muL = p.eos.m + TL * ul[1];
-        const double TR = exp(ur[0]), muR = p.eos.m + TR * ur[1];
-        okL = p2c<EK>(TL, muL, ul[2], ul[3], dL, tau, p.eos, cL, sL);
-        okR = p2c<EK>(TR, muR, ur[2], ur[3], dR, tau, p.eos, cR, sR);
-        good = okL && okR && cL[0] >= 0.0 && cR[0] >= 0.0 && cL[3] >= E_FLOOR && cR[3] >= E_FLOOR;
-    }
-    if (!good) {
-        okL = p2c<EK>(exp(yTi), w.mu[i], uxi, uyi, dL, tau, p.eos, cL, sL);
-        okR = p2c<EK>(exp(yTp), w.mu[ip], uxp, uyp, dR, tau, p.eos, cR, sR);
-    }
-    // _floor_prim_2d! (src2d/rhs2d.jl:259-270): vacuum side at the energy floor
-    Diss zL = dL, zR = dR;
-    if (!okL) {
-        cL[0] = cL[1] = cL[2] = 0.0; cL[3] = E_FLOOR;
-        sL = {T_MIN, 0.0, 0.0, 0.0, E_FLOOR, 0.0, eos_cs2<EK>(T_MIN, p.eos)};
-        zL = {0, 0, 0, 0, 0, 0, 0};
-    }
-    if (!okR) {
-        cR[0] = cR[1] = cR[2] = 0.0; cR[3] = E_FLOOR;
-        sR = {T_MIN, 0.0, 0.0, 0.0, E_FLOOR, 0.0, eos_cs2<EK>(T_MIN, p.eos)};
-        zR = {0, 0, 0, 0, 0, 0, 0};
-    }
-    // hlle_flux_2d! (src2d/fluxes2d.jl:121-157)
-    double FL[4], FR[4];
-    flux4(sL, zL, tau, dir, FL);
-    flux4(sR, zR, tau, dir, FR);
-    double lmL, lpL, lmR, lpR;
-    wavespeeds(sL, zL.Pi, dir == 0 ? zL.pixx : zL.piyy, dir, &lmL, &lpL);
-    wavespeeds(sR, zR.Pi, dir == 0 ? zR.pixx : zR.piyy, dir, &lmR, &lpR);
-    const double a_l = jmin(lmL, lmR), a_r = jmax(lpL, lpR);
-    if (WRITE) {
-        const int mode = a_l >= 0 ? 0 : (a_r <= 0 ? 1 : 2);
-        const double inv = 1 / (a_r - a_l + TINY);
-#pragma unroll
-        for (int a = 0; a < 4; ++a)
-            out[a * ostride] = mode == 0 ? FL[a] : (mode == 1 ? FR[a] : (a_r * FL[a] - a_l * FR[a] + a_r * a_l * (cR[a] - cL[a])) * inv);
-        if (p.nv > 4) {
-            // dissipative rows: zero physical flux, HLLE jump of the stored values
-            // (zero on a floor side); charm second-moment rows never enter the face states
-            const double jf = mode == 2 ? a_r * a_l : 0.0;
-            auto row = [&](int idx, double l, double r) {
-                if (idx >= 0) out[idx * ostride] = mode == 2 ? (a_r * 0.0 - a_l * 0.0 + jf * (r - l)) * inv : 0.0;
-            };
-            row(p.iNux, zL.nux, zR.nux); row(p.iNuy, zL.nuy, zR.nuy); row(p.iPi, zL.Pi, zR.Pi);
-            row(p.iPixx, zL.pixx, zR.pixx); row(p.iPixy, zL.pixy, zR.pixy);
-            row(p.iPiyy, zL.piyy, zR.piyy); row(p.iPieta, zL.pieta, zR.pieta);
-        }
-    }
-    const double aa = jmax(fabs(a_l), fabs(a_r));
-    return aa;
-}
-
-// ---- cell update: floors, S and nu bounds, admissibility -----------------------------
-struct CellState { double D, Sx, Sy, E, nux, nuy, Pi, pixx, pixy, piyy, pieta; };
-
-__device__ __forceinline__ void floors_cell(CellState& s, const Params& p, int ix, int iy) {
-    // enforce_floors_2d! (src2d/floors2d.jl:24-104)
-    const double Emin = E_FLOOR;
-    const double Evac = jmax(Emin, p.Evac);
-    auto blank = [&]() {
-        s.Sx = 0.0; s.Sy = 0.0; s.D = 0.0;
-        s.nux = 0.0; s.nuy = 0.0; s.Pi = 0.0; s.pixx = 0.0; s.pixy = 0.0; s.piyy = 0.0; s.pieta = 0.0;
-    };
-    if (p.has_rdom) {
-        const double x = p.xmin + (ix + 1 - p.ng - 0.5) * p.dx, y = p.ymin + (iy + 1 - p.ng - 0.5) * p.dy;
-        if (x * x + y * y > p.rdom2) { s.E = Emin; blank(); return; }
-    }
-    if (!isf(s.E)) { s.E = Emin; blank(); return; }
-    if (s.E < Evac) { s.E = Emin; blank(); return; }
-    if (s.E <= 1e6 * Emin) blank();
-    if (!isf(s.D) || s.D < 0.0) s.D = 0.0;
-    if (!isf(s.Sx)) s.Sx = 0.0;
-    if (!isf(s.Sy)) s.Sy = 0.0;
-    if (!isf(s.nux)) s.nux = 0.0;
-    if (!isf(s.nuy)) s.nuy = 0.0;
-    if (!isf(s.Pi)) s.Pi = 0.0;
-    if (!isf(s.pixx)) s.pixx = 0.0;
-    if (!isf(s.pixy)) s.pixy = 0.0;
-    if (!isf(s.piyy)) s.piyy = 0.0;
-    if (!isf(s.pieta)) s.pieta = 0.0;
-}
-__device__ __forceinline__ double peff_cell(const CellState& s, const Params& p, double P, bool useP) {
-    double Peff = 0.0;
-    if (useP && isf(P)) Peff = jmax(P, 0.0);
-    if (p.hasPi && isf(s.Pi)) Peff += fabs(s.Pi);
-    if (p.hasShear) {
-        if (isf(s.pixx)) Peff += fabs(s.pixx);
-        if (isf(s.pixy)) Peff += fabs(s.pixy);
-        if (isf(s.piyy)) Peff += fabs(s.piyy);
-        if (isf(s.pieta)) Peff += fabs(s.pieta);
-    }
-    return Peff;
-}
-__device__ __forceinline__ void bounds_cell(CellState& s, const Params& p, double P, bool useP, double tau) {
-    // enforce_S_energy_constraint_2d! (src2d/floors2d.jl:118-162)
-    if (isf(s.E) && isf(s.Sx) && isf(s.Sy) && !(s.E <= 0)) {
-        const double Smax = CHI_SRE * (s.E + peff_cell(s, p, P, useP));
-        const double aS = hypot(s.Sx, s.Sy);
-        if (aS > Smax) { const double sc = Smax / (aS + TINY); s.Sx = s.Sx * sc; s.Sy = s.Sy * sc; }
-    }
-    // enforce_nu_charge_constraint_2d! (src2d/floors2d.jl:183-213)
-    if (p.hasNu) {
-        const double Dt = s.D / tau;
-        if (isf(Dt) && isf(s.nux) && isf(s.nuy)) {
-            if (Dt <= 0.0) { s.nux = 0.0; s.nuy = 0.0; }
-            else {
-                const double cap = 0.999 * Dt, an = hypot(s.nux, s.nuy);
-                if (an > cap) { const double sc = cap / (an + TINY); s.nux = s.nux * sc; s.nuy = s.nuy * sc; }
-            }
-        }
-    }
-}
-// cell_admissible_2d (src2d/floors2d.jl:281-337), cheap filters only:
-// returns 1 admissible, 0 inadmissible, 2 = within the margin of the causal bound,
-// where the reference settles it by inverting the state (admissible_newton below).
-__device__ __forceinline__ int admissible_filters(const CellState& s, const Params& p, bool use_work, int ok,
-                                                  double vx, double vy, double P) {
-    const double Emin = E_FLOOR;
-    if (!(isf(s.E) && isf(s.Sx) && isf(s.Sy) && isf(s.D))) return 0;
-    if (!(s.E >= Emin)) return 0;
-    if (!(s.D >= 0.0)) return 0;
-    if (p.hasNu && !(isf(s.nux) && isf(s.nuy))) return 0;
-    if (p.hasPi && !isf(s.Pi)) return 0;
-    if (p.hasShear && !(isf(s.pixx) && isf(s.pixy) && isf(s.piyy) && isf(s.pieta))) return 0;
-    if (s.E <= 1e6 * Emin) return 1;
-    if (use_work && ok) {
-        if (!(isf(vx) && isf(vy))) return 0;
-        if (!((vx * vx + vy * vy) < (1 - 1e-12) * (1 - 1e-12))) return 0;
-    }
-    if (hypot(s.Sx, s.Sy) <= SR_MARGIN * CHI_SRE * (s.E + peff_cell(s, p, P, use_work))) return 1;
-    return 2;
-}
-template <int EK, bool NCH>
-__device__ __forceinline__ bool admissible_newton(const CellState& s, const Params& p, double tau) {
-    const Diss d = {s.nux, s.nuy, s.Pi, s.pixx, s.pixy, s.piyy, s.pieta};
-    const double nan = __longlong_as_double(0x7ff8000000000000LL);
-    PrimRes r = cons_to_prim<EK, NCH>(make_cell<NCH>(safe_div(s.D, tau), s.Sx, s.Sy, s.E, d), nan, nan, nan, nan,
-                                      p.eos);
-    return r.ok || r.reason == PRR_VACUUM;
-}
-
-__device__ __forceinline__ CellState load_cell(const double* __restrict__ U, const Params& p, int i) {
-    CellState s;
-    s.D = U[i]; s.Sx = U[P_ + i]; s.Sy = U[2 * P_ + i]; s.E = U[3 * P_ + i];
-    const Diss d = load_diss(U, p, i);
-    s.nux = d.nux; s.nuy = d.nuy; s.Pi = d.Pi; s.pixx = d.pixx; s.pixy = d.pixy; s.piyy = d.piyy; s.pieta = d.pieta;
-    return s;
-}
-__device__ __forceinline__ void store_cell(double* __restrict__ U, const Params& p, int i, const CellState& s) {
-    U[i] = s.D; U[P_ + i] = s.Sx; U[2 * P_ + i] = s.Sy; U[3 * P_ + i] = s.E;
-    if (p.hasNu) { U[p.iNux * P_ + i] = s.nux; U[p.iNuy * P_ + i] = s.nuy; }
-    if (p.hasPi) U[p.iPi * P_ + i] = s.Pi;
-    if (p.hasShear) {
-        U[p.iPixx * P_ + i] = s.pixx; U[p.iPixy * P_ + i] = s.pixy;
-        U[p.iPiyy * P_ + i] = s.piyy; U[p.iPieta * P_ + i] = s.pieta;
-    }
-}
-
-// ---- the fused face / update kernel ---------------------------------------------------
-template <int EK, bool NCH, int TX, int TY>
+template <int EK, bool NCH, int TX, int TY, int MODE>
 __global__ __launch_bounds__(TX* TY) void k_faces(FaceK K) {
     extern __shared__ double sF[];
-    constexpr int NT = TX * TY;
-    constexpr int NFX = (TX + 1) * TY, NFY = TX * (TY + 1), NF = NFX + NFY;
-    const Params& p = K.p;
-    const int tid = threadIdx.x;
-    double amax = 0.0;
-    if ((int)blockIdx.x < K.nmain) {
-        const int b = xcd_remap(blockIdx.x, K.nmain);
-        const int tx_t = b / K.tiles_y, ty_t = b - tx_t * K.tiles_y;
-        const int x0 = p.ng + tx_t * TX, y0 = p.ng + ty_t * TY;
-        const int nxt = min(TX, p.ng + p.Nx - x0), nyt = min(TY, p.ng + p.Ny - y0);
-        const uint8_t* force = K.f.force;
-        for (int f = tid; f < NF; f += NT) {
-            // one call site for both directions keeps a single copy of the face code
-            int i = 0, dir = 0, k0 = 0, st = 0;
-            bool valid;
-            if (f < NFX) {
-                const int fx = f / TY, fy = f - fx * TY;
-                valid = fx <= nxt && fy < nyt;
-                k0 = x0 + fx - 1; i = k0 * p.Nytot + y0 + fy; st = p.Nytot;
-            } else {
-                const int g = f - NFX, gx = g / (TY + 1), gy = g - gx * (TY + 1);
-                valid = gx < nxt && gy <= nyt;
-                k0 = y0 + gy - 1; i = (x0 + gx) * p.Nytot + k0; dir = 1; st = 1;
-            }
-            if (valid) {
-                const bool forced = force && (force[i] || force[i + st]);
-                double a = face<EK, NCH, true>(K, i, dir, k0, forced, sF + f, NF);
-                amax = a > amax ? a : amax;
-            }
-        }
-        __syncthreads();
-        // ---- per cell: divergence + source, then the RK stage update ----
-        const int cx = tid / TY, cy = tid - cx * TY;
-        if (cx < nxt && cy < nyt) {
-            const int ix = x0 + cx, iy = y0 + cy, i = ix * p.Nytot + iy;
-            const int fL = cx * TY + cy, fR = fL + TY, gB = NFX + cx * (TY + 1) + cy, gT = gB + 1;
-            const double idx = 1 / p.dx, idy = 1 / p.dy, itau = safe_inv(K.f.tau);
-            const double* __restrict__ Ub = K.f.Ub;
-            CellState b = load_cell(Ub, p, i);
-            const double Pw = K.w.P[i];
-            auto du = [&](int a) {
-                return -(sF[a * NF + fR] - sF[a * NF + fL]) * idx - (sF[a * NF + gT] - sF[a * NF + gB]) * idy;
-            };
-            // geometric source (src2d/fluxes2d.jl:166-177)
-            const double S1 = -b.Sx * itau, S2 = -b.Sy * itau, S3 = -(b.E + (Pw + b.Pi) + b.pieta) * itau;
-            if (K.f.mode == 0) {
-                double* dU = K.f.dUout;
-                dU[i] = du(0) + 0.0;
-                dU[P_ + i] = du(1) + S1;
-                dU[2 * P_ + i] = du(2) + S2;
-                dU[3 * P_ + i] = du(3) + S3;
-                const int nvf = p.iPQ0 >= 0 ? p.iPQ0 : p.nv;
-                for (int a = 4; a < nvf; ++a) dU[a * P_ + i] = du(a) + 0.0;
-                for (int a = nvf; a < p.nv; ++a) dU[a * P_ + i] = 0.0;
-            } else {
-                const double D = K.f.D, wo = K.f.wold, wn = K.f.wnew;
-                const int ct = K.f.combine;
-                const double* __restrict__ Uo = K.f.Uold;
-                auto comb = [&](int a, double in, double k) {
-                    if (ct == 0) return in + D * k;
-                    const double old = Uo[a * P_ + i];
-                    if (ct == 1) return wo * old + wn * (in + D * k);
-                    return (old + 2.0 * (in + D * k)) / 3.0;
-                };
-                CellState s;
-                s.D = comb(0, b.D, du(0) + 0.0);
-                s.Sx = comb(1, b.Sx, du(1) + S1);
-                s.Sy = comb(2, b.Sy, du(2) + S2);
-                s.E = comb(3, b.E, du(3) + S3);
-                s.nux = p.hasNu ? comb(p.iNux, b.nux, du(p.iNux) + 0.0) : 0.0;
-                s.nuy = p.hasNu ? comb(p.iNuy, b.nuy, du(p.iNuy) + 0.0) : 0.0;
-                s.Pi = p.hasPi ? comb(p.iPi, b.Pi, du(p.iPi) + 0.0) : 0.0;
-                s.pixx = p.hasShear ? comb(p.iPixx, b.pixx, du(p.iPixx) + 0.0) : 0.0;
-                s.pixy = p.hasShear ? comb(p.iPixy, b.pixy, du(p.iPixy) + 0.0) : 0.0;
-                s.piyy = p.hasShear ? comb(p.iPiyy, b.piyy, du(p.iPiyy) + 0.0) : 0.0;
-                s.pieta = p.hasShear ? comb(p.iPieta, b.pieta, du(p.iPieta) + 0.0) : 0.0;
-                if (p.iPQ0 >= 0) {
-                    // charm second-moment rows: never in the face states, zero flux
-                    for (int k = 0; k < 5; ++k) {
-                        const int a = p.iPQ0 + k;
-                        K.f.Uout[a * P_ + i] = comb(a, Ub[a * P_ + i], 0.0);
-                    }
-                }
-                // sanitize_stage_2d! at tau_state (src2d/floors2d.jl:218-227)
-                floors_cell(s, p, ix, iy);
-                bounds_cell(s, p, Pw, true, K.f.tau_state);
-                store_cell(K.f.Uout, p, i, s);
-                // state_ok_2d / mark_bad_2d! (src2d/floors2d.jl:281-363); near-bound
-                // cells are flagged 2 and inverted by k_admissible_pending
-                const int adm = admissible_filters(s, p, true, K.w.ok[i], K.w.vx[i], K.w.vy[i], Pw);
-                K.w.bad[i] = adm == 1 ? 0 : (adm == 0 ? 1 : 2);
-                if (adm == 0) atomicAdd(&K.sc->nbad, 1ull);
-                if (adm == 2) atomicAdd(&K.sc->npending, 1ull);
-            }
-        }
-    } else {
-        // faces outside every interior cell's stencil: the reference evaluates them
-        // too and their signal speeds enter amax (src2d/rhs2d.jl:290-367)
-        const int ng = p.ng, Nx = p.Nx, Ny = p.Ny, Nxt = p.Nxtot, Nyt = p.Nytot;
-        int t = (blockIdx.x - K.nmain) * NT + tid;
-        const int r1 = ng * (Nxt - 1), r3 = Ny * (ng - 1), r5 = ng * (Nyt - 1), r7 = Nx * (ng - 1);
-        int ix = -1, iy = -1, dir = 0;
-        if (t < r1) { ix = t / ng; iy = t % ng; }
-        else if ((t -= r1) < r1) { ix = t / ng; iy = ng + Ny + t % ng; }
-        else if ((t -= r1) < r3) { iy = ng + t / (ng - 1); ix = t % (ng - 1); }
-        else if ((t -= r3) < r3) { iy = ng + t / (ng - 1); ix = ng + Nx + t % (ng - 1); }
-        else if ((t -= r3) < r5) { dir = 1; ix = t / (Nyt - 1); iy = t % (Nyt - 1); }
-        else if ((t -= r5) < r5) { dir = 1; ix = ng + Nx + t / (Nyt - 1); iy = t % (Nyt - 1); }
-        else if ((t -= r5) < r7) { dir = 1; ix = ng + t / (ng - 1); iy = t % (ng - 1); }
-        else if ((t -= r7) < r7) { dir = 1; ix = ng + t / (ng - 1); iy = ng + Ny + t % (ng - 1); }
-        if (ix >= 0) {
-            const int i = ix * Nyt + iy;
-            double a = face<EK, NCH, false>(K, i, dir, dir == 0 ? ix : iy, false, nullptr, 0);
-            amax = a > amax ? a : amax;
-        }
-    }
-    amax = wave_max_pos(amax);
-    if ((tid & 63) == 0) atomic_max_pos(&K.sc->amax_bits, amax);
+    faces_body<EK, NCH, TX, TY, MODE>(K, blockIdx.x, sF);
+}
+
+template <int EK, bool NCH>
+__global__ __launch_bounds__(256) void k_admissible_pending(const double* U, Params p, double tau, uint8_t* bad,
+                                                            DevScalars* sc) {
+    pending_cell<EK, NCH>(U, p, tau, bad, sc, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+__global__ void k_history(DevWork w, int n) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) history_cell(w, t);
+}
+
+__global__ __launch_bounds__(256) void k_project_shear(double* U, Params p, DevWork w, int correct, double* part) {
+    project_shear_block(U, p, w, correct, blockIdx.x * blockDim.x + threadIdx.x, part + 2 * blockIdx.x);
 }
 
 // ---- MOOD mask dilation (src2d/floors2d.jl:369-383) ------------------------------------
@@ -676,20 +92,6 @@ __global__ __launch_bounds__(256) void k_admissible(const double* U, Params p, D
     if (bad) bad[i] = adm ? 0 : 1;
     if (!adm) atomicAdd(&sc->nbad, 1ull);
 }
-// the pending near-bound cells of a stage: settle each by a cold inversion
-template <int EK, bool NCH>
-__global__ __launch_bounds__(256) void k_admissible_pending(const double* U, Params p, double tau, uint8_t* bad,
-                                                            DevScalars* sc) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= p.Nx * p.Ny) return;
-    const int ix = p.ng + t / p.Ny, iy = p.ng + t % p.Ny, i = ix * p.Nytot + iy;
-    if (bad[i] != 2) return;
-    CellState s = load_cell(U, p, i);
-    const bool adm = admissible_newton<EK, NCH>(s, p, tau);
-    bad[i] = adm ? 0 : 1;
-    if (!adm) atomicAdd(&sc->nbad, 1ull);
-}
-
 // ---- IC forward map: set_cell! on every cell (main2D.jl:142-154) -------------------------
 struct InitK { const double* a[10]; };
 template <int EK>
@@ -721,13 +123,6 @@ __global__ __launch_bounds__(256) void k_init(double* U, Params p, double tau0, 
 __global__ void k_fill(double* a, double v, int64_t n) {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) a[t] = v;
-}
-// history snapshot before the step (main2D.jl:458-463)
-__global__ void k_history(DevWork w, int n) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    w.alpha_prev[t] = w.alpha[t];
-    w.T_prev[t] = exp(w.yT[t]);
 }
 // reference AoS (U[a + nv*i]) <-> device SoA
 __global__ void k_layout(const double* __restrict__ src, double* __restrict__ dst, Params p, int to_soa) {
@@ -762,35 +157,6 @@ __global__ __launch_bounds__(256) void k_diag(const double* U, Params p, DevWork
     }
     if (threadIdx.x == 0) { part[3 * blockIdx.x] = s0[0]; part[3 * blockIdx.x + 1] = s1[0]; part[3 * blockIdx.x + 2] = s2[0]; }
 }
-// project_shear! (main2D.jl:332-358): per-block (max, sum) of the residual
-__global__ __launch_bounds__(256) void k_project_shear(double* U, Params p, DevWork w, int correct, double* part) {
-    __shared__ double s0[256], s1[256];
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    double rel = 0.0;
-    if (t < p.Nx * p.Ny) {
-        const int ix = p.ng + t / p.Ny, iy = p.ng + t % p.Ny, i = ix * p.Nytot + iy;
-        const double ux = w.ux[i], uy = w.uy[i], ut = sqrt(1 + ux * ux + uy * uy);
-        const double pixx = U[p.iPixx * P_ + i], pixy = U[p.iPixy * P_ + i], piyy = U[p.iPiyy * P_ + i];
-        const double pieta = U[p.iPieta * P_ + i];
-        double tt, tx, ty;
-        shear_tau_row(ux, uy, ut, pixx, pixy, piyy, &tt, &tx, &ty);
-        const double R = -tt + pixx + piyy + pieta;
-        const double scale = jmax(jmax(jmax(jmax(fabs(tt), fabs(pixx)), fabs(piyy)), fabs(pieta)), fabs(pixy));
-        rel = scale > TINY ? fabs(R) / scale : 0.0;
-        if (correct) U[p.iPieta * P_ + i] = tt - pixx - piyy;
-    }
-    s0[threadIdx.x] = rel; s1[threadIdx.x] = rel;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) {
-            s0[threadIdx.x] = jmax(s0[threadIdx.x], s0[threadIdx.x + o]);
-            s1[threadIdx.x] += s1[threadIdx.x + o];
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) { part[2 * blockIdx.x] = s0[0]; part[2 * blockIdx.x + 1] = s1[0]; }
-}
-
 // ---- launchers -------------------------------------------------------------------------------
 #define EOS_DISPATCH(c, BODY)                                                    \
     do {                                                                         \
@@ -825,24 +191,24 @@ template <int EK, bool NCH, int TX, int TY>
 static hipError_t faces_tx(Ctx& c, const FaceLaunch& f) {
     const Params& p = c.p;
     FaceK K;
-    K.p = p; K.f = f; K.w = c.w; K.sc = c.sc;
-    const int tiles_x = (p.Nx + TX - 1) / TX;
-    K.tiles_y = (p.Ny + TY - 1) / TY;
-    K.nmain = tiles_x * K.tiles_y;
-    const int ng = p.ng;
-    const int nextra = 2 * ng * (p.Nxtot - 1) + 2 * p.Ny * (ng - 1) + 2 * ng * (p.Nytot - 1) + 2 * p.Nx * (ng - 1);
-    const int nb = K.nmain + nblk(nextra, TX * TY);
+    K.p = p; K.f = f; K.w = c.w; K.sc = c.sc; K.remap = 1;
+    int nb;
+    face_grid<TX, TY>(p, &K.tiles_y, &K.nmain, &nb);
     constexpr int NF = (TX + 1) * TY + TX * (TY + 1);
     const int nvf = p.iPQ0 >= 0 ? p.iPQ0 : p.nv;   // rows with a face flux
     const size_t lds = sizeof(double) * (size_t)nvf * NF;
     static size_t lds_set = 0;
     if (lds > 65536 && lds > lds_set) {
-        hipError_t e = hipFuncSetAttribute((const void*)k_faces<EK, NCH, TX, TY>,
+        hipError_t e = hipFuncSetAttribute((const void*)k_faces<EK, NCH, TX, TY, 0>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute((const void*)k_faces<EK, NCH, TX, TY, 1>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
         lds_set = lds;
     }
-    hipLaunchKernelGGL((k_faces<EK, NCH, TX, TY>), dim3(nb), dim3(TX * TY), lds, c.stream, K);
+    if (f.mode == 0) hipLaunchKernelGGL((k_faces<EK, NCH, TX, TY, 0>), dim3(nb), dim3(TX * TY), lds, c.stream, K);
+    else hipLaunchKernelGGL((k_faces<EK, NCH, TX, TY, 1>), dim3(nb), dim3(TX * TY), lds, c.stream, K);
     return hipGetLastError();
 }
 

@@ -252,10 +252,8 @@ __device__ __forceinline__ double alpha_grad_dir(const double* __restrict__ a, i
 }
 
 template <int EK>
-__global__ __launch_bounds__(256) void k_relax(const double* __restrict__ Q, double* __restrict__ U, Params p,
-                                               DevWork w, double tau, double D) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= p.Ntot) return;
+__device__ __forceinline__ void relax_cell(const double* __restrict__ Q, double* __restrict__ U, const Params& p,
+                                           const DevWork& w, double tau, double D, int i) {
     const int ix = i / p.Nytot, iy = i - ix * p.Nytot;
     const bool interior = ix >= p.ng && ix < p.ng + p.Nx && iy >= p.ng && iy < p.ng + p.Ny;
     const double ux = w.ux[i], uy = w.uy[i];
@@ -550,6 +548,48 @@ __global__ __launch_bounds__(256) void k_relax(const double* __restrict__ Q, dou
         w.nux_prev[i] = U[p.iNux * P_ + i];
         w.nuy_prev[i] = U[p.iNuy * P_ + i];
     }
+}
+
+template <int EK>
+__global__ __launch_bounds__(256) void k_relax(const double* __restrict__ Q, double* __restrict__ U, Params p,
+                                               DevWork w, double tau, double D) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < p.Ntot) relax_cell<EK>(Q, U, p, w, tau, D, i);
+}
+
+// the ensemble form: every event whose step was accepted in this replay
+typedef const __attribute__((address_space(4))) EvView* CEvR;
+typedef const __attribute__((address_space(4))) EvCtl* CCtlR;
+template <int EK>
+__global__ __launch_bounds__(256) void kb_relax(BatchRun R) {
+    const int e = __builtin_amdgcn_readfirstlane(((const __attribute__((address_space(4))) int*)R.map[KB_CELLS])[blockIdx.x]);
+    CCtlR c = (CCtlR)(R.ctl + e);
+    if (!c->accepted) return;
+    CEvR E = (CEvR)(R.ev + e);
+    Params p = R.p0;
+    p.Nx = E->p.Nx; p.Ny = E->p.Ny; p.Nxtot = E->p.Nxtot; p.Nytot = E->p.Nytot; p.Ntot = E->p.Ntot;
+    p.pitch = E->p.pitch; p.dx = E->p.dx; p.dy = E->p.dy; p.xmin = E->p.xmin; p.ymin = E->p.ymin;
+    DevWork w;
+    w.yT = E->w.yT; w.mu = E->w.mu; w.alpha = E->w.alpha; w.ux = E->w.ux; w.uy = E->w.uy;
+    w.P = E->w.P; w.n = E->w.n; w.e = E->w.e; w.ok = E->w.ok;
+    w.ux_prev = E->w.ux_prev; w.uy_prev = E->w.uy_prev; w.alpha_prev = E->w.alpha_prev; w.T_prev = E->w.T_prev;
+    w.nux_prev = E->w.nux_prev; w.nuy_prev = E->w.nuy_prev; w.gradAx = E->w.gradAx; w.gradAy = E->w.gradAy;
+    w.phi = E->w.phi; w.vx = E->w.vx; w.vy = E->w.vy; w.bad = E->w.bad; w.force = E->w.force; w.ftmp = E->w.ftmp;
+    w.x0_yT = E->w.x0_yT; w.x0_phi = E->w.x0_phi; w.x0_ux = E->w.x0_ux; w.x0_uy = E->w.x0_uy;
+    w.shear_res = E->w.shear_res;
+    const int i = (blockIdx.x - E->b0[KB_CELLS]) * 256 + threadIdx.x;
+    const int cur = c->cur;
+    if (i < p.Ntot) relax_cell<EK>(E->U[cur & 3], E->U[(cur + 1) & 3], p, w, c->tau, c->D, i);
+}
+
+hipError_t launch_batch_relax(Batch& b) {
+    if (b.p0.eos.kind == 1)
+        hipLaunchKernelGGL((kb_relax<1>), dim3(b.nblocks[KB_CELLS]), dim3(256), 0, b.stream, b.run);
+    else if (b.p0.eos.has_charge)
+        hipLaunchKernelGGL((kb_relax<2>), dim3(b.nblocks[KB_CELLS]), dim3(256), 0, b.stream, b.run);
+    else
+        hipLaunchKernelGGL((kb_relax<0>), dim3(b.nblocks[KB_CELLS]), dim3(256), 0, b.stream, b.run);
+    return hipGetLastError();
 }
 
 hipError_t launch_relax(Ctx& c, double* U, double tau, double dtau) {

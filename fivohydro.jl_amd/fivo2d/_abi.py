@@ -19,6 +19,8 @@ FIVO_BC_OUTFLOW = 0
 FIVO_BC_PERIODIC = 1
 FIVO_SSPRK2 = 2
 FIVO_SSPRK3 = 3
+FIVO_BATCH_NTILES = 4
+BATCH_TILES = ((4, 16), (8, 16), (8, 32), (16, 16))   # candidate face tiles, FIVO_BATCH_NTILES of them
 
 # Terms field order of src/terms.jl:82-106 -> bit index
 TERM_NAMES = (
@@ -170,7 +172,16 @@ def load_hip():
     if _HIP is None:
         lib = Lib(HIP_LIB_PATH, "fivo_")
         for name, res, args in (("kernel_time", C.c_int64, [C.c_void_p, C.c_int, _DP]),
-                                ("enable_timing", C.c_int, [C.c_void_p, C.c_int])):
+                                ("enable_timing", C.c_int, [C.c_void_p, C.c_int]),
+                                ("batch_create", C.c_int, [C.POINTER(GridDesc), C.c_int32, C.POINTER(ModelDesc),
+                                                           C.POINTER(C.c_void_p)]),
+                                ("batch_destroy", None, [C.c_void_p]),
+                                ("batch_last_error", C.c_char_p, [C.c_void_p]),
+                                ("batch_event", C.c_void_p, [C.c_void_p, C.c_int32]),
+                                ("batch_autotune", C.c_int, [C.c_void_p, C.c_double, C.POINTER(C.c_int32),
+                                                             C.POINTER(C.c_int32), C.POINTER(C.c_float)]),
+                                ("batch_set_tile", C.c_int, [C.c_void_p, C.c_int32]),
+                                ("batch_run", C.c_int, [C.c_void_p, C.POINTER(RunOpts), C.POINTER(RunResult)])):
             f = getattr(lib.dll, "fivo_" + name)
             f.restype = res
             f.argtypes = args

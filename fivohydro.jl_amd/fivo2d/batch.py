@@ -1,0 +1,81 @@
+"""Ensembles of events over the C ABI's fivo_batch_* entry points.
+
+The reference runs fluctuating events one at a time, each through its own
+run_sim_2d! (main2D.jl:371-561). `Ensemble` holds many events on their own grids
+with one model and runs them together: every phase of the SSPRK3 step is one
+launch over all events and the step is a replayed hipGraph, while each event
+keeps its own dt, halvings and stopping time. `ensemble.events[k]` is a
+`Solver` over event k's context, for initial conditions and readout.
+"""
+import ctypes as C
+
+from . import _abi
+from .solver import Solver, FivoError
+
+
+class Ensemble:
+    def __init__(self, grids, model, lib=None):
+        self.lib = lib if lib is not None else _abi.load_hip()
+        self.grids = list(grids)
+        self.model = model
+        n = len(self.grids)
+        gd = (_abi.GridDesc * n)(*[g.desc() for g in self.grids])
+        self._md = model.desc()
+        b = C.c_void_p()
+        rc = self.lib.batch_create(gd, n, C.byref(self._md), C.byref(b))
+        if rc != 0 or not b.value:
+            raise FivoError(f"batch_create failed ({rc}): {self._err(None)}")
+        self.b = b
+        self.events = [Solver.borrowed(g, model, self.lib, self.lib.batch_event(b, k))
+                       for k, g in enumerate(self.grids)]
+        self.tile = None
+
+    def _err(self, b):
+        s = self.lib.batch_last_error(b)
+        return s.decode() if s else ""
+
+    def _chk(self, rc, what):
+        if rc < 0:
+            raise FivoError(f"{what} failed ({rc}): {self._err(self.b)}")
+        return rc
+
+    def close(self):
+        if getattr(self, "b", None) is not None and self.b.value:
+            for s in self.events:
+                s.ctx = None
+            self.lib.batch_destroy(self.b)
+            self.b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def autotune(self, tau):
+        """Time the fused face kernel for every candidate tile on the current
+        states at tau and keep the fastest: returns ((tx, ty), {tile: ms})."""
+        tx, ty = C.c_int32(0), C.c_int32(0)
+        ms = (C.c_float * _abi.FIVO_BATCH_NTILES)()
+        self._chk(self.lib.batch_autotune(self.b, float(tau), C.byref(tx), C.byref(ty), ms), "batch_autotune")
+        self.tile = (tx.value, ty.value)
+        return self.tile, {_abi.BATCH_TILES[k]: ms[k] for k in range(_abi.FIVO_BATCH_NTILES)}
+
+    def set_tile(self, tile):
+        k = _abi.BATCH_TILES.index(tuple(tile))
+        self._chk(self.lib.batch_set_tile(self.b, k), "batch_set_tile")
+        self.tile = tuple(tile)
+
+    def run_sim_2d(self, tau0=0.4, taufinal=13.0, CFL=0.2, CFLtau=0.05, bc=_abi.FIVO_BC_OUTFLOW,
+                   reset_history=True, max_steps=2_000_000, max_halve=12):
+        """run_sim_2d! (main2D.jl:371-561) for every event with SSPRK3; one result dict per event."""
+        o = _abi.RunOpts()
+        o.tau0, o.taufinal, o.cfl, o.cfltau, o.dump_dt = tau0, taufinal, CFL, CFLtau, 0.0
+        o.integrator = _abi.FIVO_SSPRK3
+        o.bc = int(bc)
+        o.max_halve, o.max_mood, o.max_steps = max_halve, 0, max_steps
+        o.reset_history = int(bool(reset_history))
+        n = len(self.grids)
+        res = (_abi.RunResult * n)()
+        self._chk(self.lib.batch_run(self.b, C.byref(o), res), "batch_run")
+        return [r.as_dict() for r in res]

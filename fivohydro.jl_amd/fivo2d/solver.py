@@ -54,10 +54,22 @@ class Solver:
         self.Ntot = self.lib.ntot(ctx)
         assert self.nv == model.layout.nv and self.Ntot == g.Ntot
 
+    @classmethod
+    def borrowed(cls, g, model, lib, ctx):
+        """A Solver over a context owned elsewhere (an event of an ensemble)."""
+        s = cls.__new__(cls)
+        s.g, s.model, s.lib = g, model, lib
+        s._gd, s._md = g.desc(), model.desc()
+        s.ctx = C.c_void_p(ctx)
+        s._owned = False
+        s.nv = lib.nvars(s.ctx)
+        s.Ntot = lib.ntot(s.ctx)
+        return s
+
     def close(self):
-        if getattr(self, "ctx", None) is not None and self.ctx.value:
+        if getattr(self, "ctx", None) is not None and self.ctx.value and getattr(self, "_owned", True):
             self.lib.destroy(self.ctx)
-            self.ctx = None
+        self.ctx = None
 
     def __del__(self):
         try:

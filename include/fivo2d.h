@@ -217,6 +217,31 @@ int fivo_run(fivo_ctx* ctx, const fivo_run_opts* o, fivo_run_result* r);
 int64_t fivo_kernel_time(const fivo_ctx* ctx, int category, double* total_ms);
 int fivo_enable_timing(fivo_ctx* ctx, int on);
 
+/* ---- ensembles of events ------------------------------------------------- */
+/* The reference runs fluctuating events one at a time, each with its own
+ * run_sim_2d! (main2D.jl:371-561). An ensemble runs many such events at once:
+ * each phase of the SSPRK3 step is one launch over every event, the step is a
+ * captured hipGraph, and every event keeps its own dt, halvings and stopping
+ * time, so each event's result is the one fivo_run gives for it alone. */
+typedef struct fivo_batch fivo_batch;
+/* nev events on their own grids, one model (build_model_2d, main2D.jl:85-123) */
+int fivo_batch_create(const fivo_grid_desc* grids, int32_t nev, const fivo_model_desc* m, fivo_batch** out);
+void fivo_batch_destroy(fivo_batch* b);
+const char* fivo_batch_last_error(const fivo_batch* b);  /* b may be NULL */
+/* the one-event context of event ev: set its state and initial conditions, read
+ * it back, or run it alone with the one-event API */
+fivo_ctx* fivo_batch_event(fivo_batch* b, int32_t ev);
+/* time the fused face kernel for every candidate tile shape on the current
+ * states at tau (after a primitive pass there) and keep the fastest; ms gets
+ * one time per candidate (FIVO_BATCH_NTILES), tx/ty the chosen tile */
+#define FIVO_BATCH_NTILES 4
+int fivo_batch_autotune(fivo_batch* b, double tau, int32_t* tx, int32_t* ty, float* ms);
+/* use candidate tile `tile` (0..FIVO_BATCH_NTILES-1) without timing */
+int fivo_batch_set_tile(fivo_batch* b, int32_t tile);
+/* run_sim_2d! for every event (SSPRK3, no dump callback): res[ev] per event.
+ * Returns FIVO_STEP_FAILED when some event failed, like fivo_run for it alone. */
+int fivo_batch_run(fivo_batch* b, const fivo_run_opts* o, fivo_run_result* res);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,104 @@
+"""GPU tests of ensembles (fivo_batch_*): every event of an ensemble must end
+where the one-event run_sim_2d! (fivo_run) takes it alone, and where the CPU
+oracle takes it (final T and u within 1e-9).
+
+The ensemble runs the same device bodies as the one-event kernels, on the same
+per-event arrays, with the same per-event dt / halving decisions made on the
+device: step counts, halvings, final times and states must agree bit for bit
+(multiply-adds fuse only within one source expression, so a body rounds the
+same way in both kernels)."""
+import numpy as np
+import pytest
+
+import fivo2d as F
+from fivo2d import ic
+from helpers import rel_diff
+
+pytestmark = pytest.mark.gpu
+
+
+def full_model():
+    """The full-sector model of the 512^2 workload (shear, bulk, diffusion, charm m2)."""
+    return F.build_model_2d(eos=F.LatticeHRGEOS(), enable_shear=True, eta_over_s=0.08, tauShear_coeff=0.2,
+                            enable_bulk=True, zeta_over_s=0.1, tauPi_coeff=15.0, enable_diff=True,
+                            kappa_coeff=0.1163, pi_clip_factor=1.0, consistent_m2=True)
+
+
+# uneven grids and hot-spot counts, like the 64-event ensemble in miniature
+EVENTS = [((40, 40), 7, 60), ((56, 48), 11, 120), ((32, 64), 13, 40)]
+
+
+def events(L=12.0):
+    out = []
+    for (nx, ny), seed, nspots in EVENTS:
+        g = F.make_grid2d(nx, ny, xmax=L, ymax=L * ny / nx)
+        out.append((g, ic.hotspot_event(g, seed=seed, nspots=nspots, sigma=0.5, Tpeak=0.45)))
+    return out
+
+
+RUN = dict(tau0=0.6, taufinal=0.9, CFL=0.2, CFLtau=0.05)
+
+
+def run_alone(lib, g, m, prims):
+    s = F.Solver(g, m, lib=lib)
+    s.init_from_prims(RUN["tau0"], **prims)
+    r = s.run_sim_2d(integrator="ssprk3", **RUN)
+    return r, s.get_state(), s.get_work(("yT", "ux", "uy"))
+
+
+def run_ensemble(hip, m, evs, tile=None, autotune=False):
+    ens = F.Ensemble([g for g, _ in evs], m, lib=hip)
+    for s, (g, prims) in zip(ens.events, evs):
+        s.init_from_prims(RUN["tau0"], **prims)
+    if autotune:
+        tile, ms = ens.autotune(RUN["tau0"])
+        assert tile in F._abi.BATCH_TILES and all(v > 0 for v in ms.values())
+    elif tile is not None:
+        ens.set_tile(tile)
+    res = ens.run_sim_2d(**RUN)
+    out = [(r, s.get_state(), s.get_work(("yT", "ux", "uy"))) for r, s in zip(res, ens.events)]
+    ens.close()
+    return out
+
+
+@pytest.mark.parametrize("tile", [(4, 16), (8, 32)])
+def test_ensemble_equals_one_event_runs(hip, tile):
+    m = full_model()
+    evs = events()
+    alone = [run_alone(hip, g, m, prims) for g, prims in evs]
+    together = run_ensemble(hip, m, evs, tile=tile)
+    for k, ((ra, Ua, Wa), (rb, Ub, Wb)) in enumerate(zip(alone, together)):
+        print(f"event {k}: alone nsteps={ra['nsteps']} tau={ra['tau']:.6f} halvings={ra['nhalvings']}; "
+              f"ensemble nsteps={rb['nsteps']} tau={rb['tau']:.6f}", flush=True)
+        assert ra["ok"] and rb["ok"]
+        for key in ("nsteps", "nhalvings", "nprimfail", "nvacuum", "nrhs"):
+            assert ra[key] == rb[key], (k, key, ra[key], rb[key])
+        assert ra["tau"] == rb["tau"]
+        r = rel_diff(Ub, Ua)
+        nbits = int(np.sum(Ua != Ub))
+        print(f"    state: {nbits} of {Ua.size} values differ, max rel diff per field {np.max(r):.2e}", flush=True)
+        assert nbits == 0, (k, r)
+        for f in ("yT", "ux", "uy"):
+            assert np.array_equal(Wa[f], Wb[f]), (k, f)
+        assert ra["Q1"] == rb["Q1"] and ra["max_shear_res"] == rb["max_shear_res"]
+
+
+def test_ensemble_autotuned_matches_oracle(oracle, hip):
+    """Autotuned ensemble vs the CPU oracle per event: final T and u within 1e-9."""
+    m = full_model()
+    evs = events()
+    together = run_ensemble(hip, m, evs, autotune=True)
+    g0 = [g for g, _ in evs]
+    for k, ((g, prims), (rb, Ub, Wb)) in enumerate(zip(evs, together)):
+        ro, Uo, Wo = run_alone(oracle, g, m, prims)
+        assert ro["ok"] and rb["ok"]
+        assert ro["nsteps"] == rb["nsteps"] and ro["nhalvings"] == rb["nhalvings"], (k, ro, rb)
+        sx, sy = g.interior()
+        T = lambda W: np.exp(W["yT"].reshape(g.Nxtot, g.Nytot)[sx, sy])
+        u = lambda W, f: W[f].reshape(g.Nxtot, g.Nytot)[sx, sy]
+        dT = np.max(np.abs(T(Wb) - T(Wo)) / T(Wo))
+        du = max(np.max(np.abs(u(Wb, f) - u(Wo, f))) for f in ("ux", "uy"))
+        print(f"event {k} ({g.Nx}x{g.Ny}): nsteps {rb['nsteps']} dT/T {dT:.2e} du {du:.2e}", flush=True)
+        assert dT <= 1e-9 and du <= 1e-9, (k, dT, du)
+        r = rel_diff(Ub, Uo)
+        assert np.max(r) <= 1e-7, (k, r)
