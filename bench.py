@@ -61,6 +61,95 @@ CONFIGS = {
 }
 
 
+# The 64-event ensemble: uneven grids drawn from {128..384}^2, 50-300 hot spots,
+# the full-sector model of the 512^2 workload, on [-15,15] fm.
+ENSEMBLE = dict(
+    workload="ensemble-64-events",
+    desc="64 seeded hot-spot events (sigma 0.4 fm, Woods-Saxon centres, Gamma(1) amplitudes, peak T 0.45 GeV) "
+         "on grids drawn uniformly from {128,192,256,320,384}^2 over [-15,15] fm with 50-300 spots each; "
+         "full sectors (shear + bulk + nu + charm second moment), LatticeHRG EOS, tau0 0.4 fm/c, SSPRK3, "
+         "CFLtau 0.05, CFL 0.2; plain = each event through its own run_sim_2d! call, one after another",
+    nev=64, sizes=(128, 192, 256, 320, 384), spots=(50, 300), seed=2024, L=15.0, sigma=0.4, tau0=0.4,
+    cfl=0.2, cfltau=0.05, nv=16, stages=3, relax=1,
+)
+
+
+def ensemble_events(e):
+    rng = np.random.Generator(np.random.Philox(e["seed"]))
+    sizes = rng.choice(e["sizes"], size=e["nev"])
+    nsp = rng.integers(e["spots"][0], e["spots"][1] + 1, size=e["nev"])
+    return [(int(n), int(k), 1000 + i) for i, (n, k) in enumerate(zip(sizes, nsp))]
+
+
+def full_model():
+    import fivo2d as F
+    return F.build_model_2d(eos=F.LatticeHRGEOS(), enable_shear=True, eta_over_s=0.08, tauShear_coeff=0.2,
+                            enable_bulk=True, zeta_over_s=0.1, tauPi_coeff=15.0, enable_diff=True,
+                            kappa_coeff=0.1163, pi_clip_factor=1.0, consistent_m2=True)
+
+
+def ensemble_bench(lib, steps, warmup):
+    """Plain (one run_sim_2d! per event, one event at a time) vs ensemble (all
+    events per launch, hipGraph-replayed steps, autotuned face tile) on the same
+    64 events and the same number of steps each."""
+    import torch
+    import fivo2d as F
+    e = ENSEMBLE
+    evs = ensemble_events(e)
+    m = full_model()
+    grids = [F.make_grid2d(n, n, xmax=e["L"], ymax=e["L"]) for n, _, _ in evs]
+    cells = sum(g.Nx * g.Ny for g in grids)
+    leg = dict(taufinal=1e9, CFL=e["cfl"], CFLtau=e["cfltau"])
+    out = {"workload": e["workload"], "events": e["nev"], "cells_per_step": cells, "steps": steps,
+           "description": e["desc"]}
+    ens = {}
+    for path in ("plain", "batched"):
+        x = F.Ensemble(grids, m, lib=lib)
+        for s, (n, nsp, seed) in zip(x.events, evs):
+            s.init_hotspots(e["tau0"], seed=seed, nspots=nsp, sigma=e["sigma"], Tpeak=0.45)
+        ens[path] = x
+    # plain: each event alone, one after another
+    xp = ens["plain"]
+    taus = [s.run_sim_2d(tau0=e["tau0"], max_steps=warmup, integrator="ssprk3", **leg)["tau"] for s in xp.events]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rp = [s.run_sim_2d(tau0=t, max_steps=steps, integrator="ssprk3", reset_history=False, **leg)
+          for s, t in zip(xp.events, taus)]
+    torch.cuda.synchronize()
+    wall_p = time.perf_counter() - t0
+    assert all(r["ok"] and r["nsteps"] == steps for r in rp)
+    # batched: all events per launch
+    xb = ens["batched"]
+    tile, tile_ms = xb.autotune(e["tau0"])
+    rb = xb.run_sim_2d(tau0=e["tau0"], max_steps=warmup, **leg)
+    taus_b = [r["tau"] for r in rb]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rb = xb.run_sim_2d(tau0=taus_b, max_steps=steps, reset_history=False, **leg)
+    torch.cuda.synchronize()
+    wall_b = time.perf_counter() - t0
+    assert all(r["ok"] and r["nsteps"] == steps for r in rb)
+    # same events, same steps: the final temperatures must agree
+    dT = 0.0
+    for sp, sb, g in zip(xp.events, xb.events, grids):
+        ta = np.exp(sp.get_work(("yT",))["yT"])
+        tb = np.exp(sb.get_work(("yT",))["yT"])
+        dT = max(dT, float(np.max(np.abs(tb - ta) / ta)))
+    out.update({
+        "plain": {"cell_updates_per_s": cells * steps / wall_p, "event_steps_per_s": e["nev"] * steps / wall_p,
+                  "ms_per_step_all_events": wall_p / steps * 1e3},
+        "batched": {"cell_updates_per_s": cells * steps / wall_b, "event_steps_per_s": e["nev"] * steps / wall_b,
+                    "ms_per_step_all_events": wall_b / steps * 1e3, "tile": list(tile),
+                    "tile_ms": {f"{k[0]}x{k[1]}": round(v, 4) for k, v in tile_ms.items()}},
+        "speedup_batched_vs_plain": wall_p / wall_b,
+        "max_dT_over_T_batched_vs_plain": dT,
+        "newton_iters_per_cell_update": sum(r["newton_iters"] for r in rb) / (cells * steps),
+    })
+    for x in ens.values():
+        x.close()
+    return out
+
+
 def algorithmic_bytes(cfg):
     """Bytes per cell-update: RK state nv*(2 + 3(s-1)) doubles + 8 doubles of Newton
     warm start per primitive pass (s + m passes) + relaxation (nv + (nv-4) + 12) * m."""
@@ -131,6 +220,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--config", default="c3", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--ensemble-steps", type=int, default=10)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
@@ -183,6 +274,8 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "step_achieved_GBs": algorithmic_bytes(cfg) * value / 1e9},
     }
+    if not a.no_ensemble:
+        out["ensemble"] = ensemble_bench(lib, a.ensemble_steps, 3)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cfg)
     print(json.dumps(out))

@@ -593,6 +593,38 @@ int fivo_init_from_prims(fivo_ctx* ctx, double tau0, const double* T, const doub
     return fivo_finalize_ic(ctx, tau0, bc);
 }
 
+int fivo_init_hotspots(fivo_ctx* ctx, double tau0, int32_t nspots, const double* spots, double sigma, double Tpeak,
+                       double alpha_floor, int bc, int64_t* nbad) {
+    Ctx& c = *reinterpret_cast<Ctx*>(ctx);
+    if (!spots || nspots < 1 || !(sigma > 0) || !(Tpeak > 0)) return fail(&c, FIVO_ERR_ARG, "init_hotspots: bad argument");
+    const size_t P = c.p.pitch;
+    if ((size_t)3 * nspots > (size_t)c.p.nv * P) return fail(&c, FIVO_ERR_ARG, "init_hotspots: too many spots for the grid");
+    // dU scratch: s, T, alpha; snapshot buffer: the spot table
+    double* s = c.dU;
+    double* T = c.dU + P;
+    double* al = c.dU + 2 * P;
+    double* sp = c.snap;
+    const int nblk_ = (int)((c.p.Ntot + 255) / 256);
+    if (nblk_ > 3 * c.npartials + 64) return fail(&c, FIVO_ERR_ARG, "init_hotspots: reduction buffer too small");
+    HIPCHK(&c, hipMemcpyAsync(sp, spots, sizeof(double) * 3 * nspots, hipMemcpyHostToDevice, c.stream));
+    HIPCHK(&c, launch_hotspot_sum(c, s, nspots, sp, sigma, c.partials));
+    std::vector<double> part(nblk_);
+    HIPCHK(&c, hipMemcpyAsync(part.data(), c.partials, sizeof(double) * nblk_, hipMemcpyDeviceToHost, c.stream));
+    HIPCHK(&c, hipStreamSynchronize(c.stream));
+    double smax = 0.0;
+    for (double v : part) smax = v > smax ? v : smax;
+    if (!(smax > 0)) return fail(&c, FIVO_ERR_ARG, "init_hotspots: no spot reaches the grid");
+    HIPCHK(&c, launch_hotspot_prims(c, s, T, al, smax, Tpeak, alpha_floor));
+    const double* dptr[10] = {T, al, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    HIPCHK(&c, hipMemsetAsync(c.U[c.cur], 0, sizeof(double) * (size_t)c.p.nv * P, c.stream));
+    HIPCHK(&c, hipMemsetAsync(c.sc, 0, sizeof(DevScalars), c.stream));
+    HIPCHK(&c, launch_init_prims(c, c.U[c.cur], tau0, dptr, nullptr));
+    HIPCHK(&c, hipMemcpyAsync(c.sc_host, c.sc, sizeof(DevScalars), hipMemcpyDeviceToHost, c.stream));
+    HIPCHK(&c, hipStreamSynchronize(c.stream));
+    if (nbad) *nbad = (int64_t)c.sc_host->nbad;
+    return fivo_finalize_ic(ctx, tau0, bc);
+}
+
 int fivo_apply_bc(fivo_ctx* ctx, int bc) {
     Ctx& c = *reinterpret_cast<Ctx*>(ctx);
     HIPCHK(&c, launch_bc(c, c.U[c.cur], bc));

@@ -255,7 +255,7 @@ int fivo_batch_autotune(fivo_batch* bp, double tau, int32_t* tx, int32_t* ty, fl
     return FIVO_OK;
 }
 
-int fivo_batch_run(fivo_batch* bp, const fivo_run_opts* o, fivo_run_result* res) {
+int fivo_batch_run(fivo_batch* bp, const fivo_run_opts* o, const double* tau0, fivo_run_result* res) {
     Batch& b = *reinterpret_cast<Batch*>(bp);
     if (!o || !res) return bfail(&b, FIVO_ERR_ARG, "batch_run: null argument");
     if (o->integrator != FIVO_SSPRK3) return bfail(&b, FIVO_ERR_ARG, "batch_run: ensembles step with SSPRK3");
@@ -280,7 +280,7 @@ int fivo_batch_run(fivo_batch* bp, const fivo_run_opts* o, fivo_run_result* res)
             have_hist = std::isfinite(a0) ? 1 : 0;
         }
         if ((rc = event_diag(b, c, 0, &Q0[e], nullptr, nullptr)) != FIVO_OK) return rc;
-        k.tau = o->tau0; k.D = 0.0; k.taufinal = o->taufinal; k.cur = c.cur; k.state = EV_RUNNING; k.fresh = 1;
+        k.tau = tau0 ? tau0[e] : o->tau0; k.D = 0.0; k.taufinal = o->taufinal; k.cur = c.cur; k.state = EV_RUNNING; k.fresh = 1;
         k.have_hist = have_hist; k.max_steps = o->max_steps;
     }
     if ((rc = sync_events(b)) != FIVO_OK) return rc;

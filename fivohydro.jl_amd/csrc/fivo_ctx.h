@@ -125,6 +125,9 @@ hipError_t launch_admissible(Ctx& c, const double* U, double tau, int use_work, 
 hipError_t launch_admissible_pending(Ctx& c, const double* U, double tau);
 hipError_t launch_init_prims(Ctx& c, double* U, double tau0, const double* const* prims, unsigned long long* nbad);
 hipError_t launch_fill(Ctx& c, double* p, double v, int64_t n);
+hipError_t launch_hotspot_sum(Ctx& c, double* s_out, int nspots, const double* spots, double sigma, double* part);
+hipError_t launch_hotspot_prims(Ctx& c, const double* s_in, double* T, double* alpha, double smax, double Tpeak,
+                                double alpha_floor);
 hipError_t launch_history(Ctx& c);
 hipError_t launch_reduce_diag(Ctx& c, const double* U, int what);
 hipError_t launch_project_shear(Ctx& c, double* U, int correct);

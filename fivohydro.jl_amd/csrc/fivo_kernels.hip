@@ -119,6 +119,53 @@ __global__ __launch_bounds__(256) void k_init(double* U, Params p, double tau0, 
     store_cell(U, p, i, s);
 }
 
+// ---- hot-spot initial state on the device -------------------------------------------------
+// s(x) = sum_k A_k exp(-|x - x_k|^2 / (2 sigma^2)) over every cell centre (ghosts
+// included), in spot order, each product and sum rounded separately; per-block
+// maxima into part. Then T = Tpeak cbrt(s / max s) (>= T_MIN) and
+// alpha = 2 log(s / max s) (>= alpha_floor): the project's seeded event generator.
+__global__ __launch_bounds__(256) void k_hotspot_sum(double* __restrict__ s_out, Params p, int nspots,
+                                                     const double* __restrict__ spots, double inv2s2,
+                                                     double* part) {
+    __shared__ double sm[256];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    double s = 0.0;
+    if (t < p.Ntot) {
+        const int ix = t / p.Nytot, iy = t - ix * p.Nytot;
+        const double ox = (double)(ix + 1 - p.ng) - 0.5, oy = (double)(iy + 1 - p.ng) - 0.5;
+        const double px = ox * p.dx, py = oy * p.dy;
+        const double X = p.xmin + px, Y = p.ymin + py;
+        for (int k = 0; k < nspots; ++k) {
+            const double ddx = X - spots[3 * k], ddy = Y - spots[3 * k + 1];
+            const double a2 = ddx * ddx, b2 = ddy * ddy;
+            const double d2 = a2 + b2;
+            const double arg = -d2 * inv2s2;
+            const double g = exp(arg);
+            const double term = spots[3 * k + 2] * g;
+            s = s + term;
+        }
+        s_out[t] = s;
+    }
+    sm[threadIdx.x] = t < p.Ntot ? s : 0.0;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sm[threadIdx.x] = jmax(sm[threadIdx.x], sm[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
+}
+__global__ __launch_bounds__(256) void k_hotspot_prims(const double* __restrict__ s_in, double* __restrict__ T,
+                                                       double* __restrict__ alpha, int n, double smax, double Tpeak,
+                                                       double alpha_floor) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const double s = s_in[t] / smax;
+    const double Tv = Tpeak * cbrt(s);
+    T[t] = jmax(Tv, T_MIN);
+    const double a = s > 0 ? 2.0 * log(jmax(s, 1e-300)) : alpha_floor;
+    alpha[t] = jmax(a, alpha_floor);
+}
+
 // ---- small utility kernels ------------------------------------------------------------------
 __global__ void k_fill(double* a, double v, int64_t n) {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -247,6 +294,18 @@ hipError_t launch_init_prims(Ctx& c, double* U, double tau0, const double* const
     for (int k = 0; k < 10; ++k) in.a[k] = prims[k];
     EOS_DISPATCH(c, hipLaunchKernelGGL((k_init<EK>), dim3(nblk(c.p.Ntot, 256)), dim3(256), 0, c.stream, U, c.p,
                                        tau0, in, c.sc));
+    return hipGetLastError();
+}
+
+hipError_t launch_hotspot_sum(Ctx& c, double* s_out, int nspots, const double* spots, double sigma, double* part) {
+    hipLaunchKernelGGL(k_hotspot_sum, dim3(nblk(c.p.Ntot, 256)), dim3(256), 0, c.stream, s_out, c.p, nspots, spots,
+                       1.0 / (2 * sigma * sigma), part);
+    return hipGetLastError();
+}
+hipError_t launch_hotspot_prims(Ctx& c, const double* s_in, double* T, double* alpha, double smax, double Tpeak,
+                                double alpha_floor) {
+    hipLaunchKernelGGL(k_hotspot_prims, dim3(nblk(c.p.Ntot, 256)), dim3(256), 0, c.stream, s_in, T, alpha,
+                       c.p.Ntot, smax, Tpeak, alpha_floor);
     return hipGetLastError();
 }
 

@@ -181,7 +181,9 @@ def load_hip():
                                 ("batch_autotune", C.c_int, [C.c_void_p, C.c_double, C.POINTER(C.c_int32),
                                                              C.POINTER(C.c_int32), C.POINTER(C.c_float)]),
                                 ("batch_set_tile", C.c_int, [C.c_void_p, C.c_int32]),
-                                ("batch_run", C.c_int, [C.c_void_p, C.POINTER(RunOpts), C.POINTER(RunResult)])):
+                                ("batch_run", C.c_int, [C.c_void_p, C.POINTER(RunOpts), _DP, C.POINTER(RunResult)]),
+                                ("init_hotspots", C.c_int, [C.c_void_p, C.c_double, C.c_int32, _DP, C.c_double,
+                                                            C.c_double, C.c_double, C.c_int, C.POINTER(C.c_int64)])):
             f = getattr(lib.dll, "fivo_" + name)
             f.restype = res
             f.argtypes = args

@@ -9,6 +9,8 @@ keeps its own dt, halvings and stopping time. `ensemble.events[k]` is a
 """
 import ctypes as C
 
+import numpy as np
+
 from . import _abi
 from .solver import Solver, FivoError
 
@@ -68,14 +70,21 @@ class Ensemble:
 
     def run_sim_2d(self, tau0=0.4, taufinal=13.0, CFL=0.2, CFLtau=0.05, bc=_abi.FIVO_BC_OUTFLOW,
                    reset_history=True, max_steps=2_000_000, max_halve=12):
-        """run_sim_2d! (main2D.jl:371-561) for every event with SSPRK3; one result dict per event."""
+        """run_sim_2d! (main2D.jl:371-561) for every event with SSPRK3; one result dict per event.
+        tau0: one start time, or one per event (to continue events that stopped apart)."""
+        n = len(self.grids)
+        t0s = None
+        if np.ndim(tau0) > 0:
+            t0s = np.ascontiguousarray(tau0, dtype=np.float64)
+            assert t0s.shape == (n,)
+            tau0 = float(t0s[0])
         o = _abi.RunOpts()
         o.tau0, o.taufinal, o.cfl, o.cfltau, o.dump_dt = tau0, taufinal, CFL, CFLtau, 0.0
         o.integrator = _abi.FIVO_SSPRK3
         o.bc = int(bc)
         o.max_halve, o.max_mood, o.max_steps = max_halve, 0, max_steps
         o.reset_history = int(bool(reset_history))
-        n = len(self.grids)
         res = (_abi.RunResult * n)()
-        self._chk(self.lib.batch_run(self.b, C.byref(o), res), "batch_run")
+        tp = None if t0s is None else t0s.ctypes.data_as(C.POINTER(C.c_double))
+        self._chk(self.lib.batch_run(self.b, C.byref(o), tp, res), "batch_run")
         return [r.as_dict() for r in res]

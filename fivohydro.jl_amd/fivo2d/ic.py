@@ -86,6 +86,21 @@ def _woods_saxon_centres(rng, n, R, a):
     return out[:n]
 
 
+def hotspot_spots(seed, nspots, R=6.6, a=0.55):
+    """The seeded spots of an event: centres from a 2-D Woods-Saxon (R, a) and
+    amplitudes A_k ~ Gamma(k=1), from a Philox stream keyed by `seed`."""
+    rng = np.random.Generator(np.random.Philox(seed))
+    cen = _woods_saxon_centres(rng, nspots, R, a)
+    amp = rng.gamma(1.0, 1.0, size=nspots)
+    return cen, amp
+
+
+def hotspot_table(seed, nspots, R=6.6, a=0.55):
+    """(x_k, y_k, A_k) rows of the spots, the layout fivo_init_hotspots takes."""
+    cen, amp = hotspot_spots(seed, nspots, R, a)
+    return np.ascontiguousarray(np.column_stack([cen[:, 0], cen[:, 1], amp]), dtype=np.float64)
+
+
 def hotspot_event(g, seed, nspots=150, sigma=0.5, Tpeak=0.45, R=6.6, a=0.55, eos=None,
                   alpha_floor=-30.0):
     """Seeded lumpy event on grid `g`.
@@ -97,9 +112,7 @@ def hotspot_event(g, seed, nspots=150, sigma=0.5, Tpeak=0.45, R=6.6, a=0.55, eos
     in the product approximation rho_bin ~ s^2, normalised so alpha = 0 at the
     hottest cell and floored at alpha_floor. Dissipative fields start at zero.
     """
-    rng = np.random.Generator(np.random.Philox(seed))
-    cen = _woods_saxon_centres(rng, nspots, R, a)
-    amp = rng.gamma(1.0, 1.0, size=nspots)
+    cen, amp = hotspot_spots(seed, nspots, R, a)
     X, Y = _mesh(g)
     s = np.zeros_like(X)
     inv2s2 = 1.0 / (2 * sigma * sigma)

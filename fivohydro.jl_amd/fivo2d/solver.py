@@ -140,6 +140,17 @@ class Solver:
                                            C.byref(nbad)), "init_from_prims")
         return nbad.value
 
+    def init_hotspots(self, tau0, seed, nspots=150, sigma=0.5, Tpeak=0.45, R=6.6, a=0.55, alpha_floor=-30.0,
+                      bc=_abi.FIVO_BC_OUTFLOW):
+        """The seeded hot-spot event of ic.hotspot_event, built on the device
+        (fivo_init_hotspots), then set_cell! + finalize_ic! (main2D.jl:142-171)."""
+        from .ic import hotspot_table
+        tab = hotspot_table(seed, nspots, R, a)
+        nbad = C.c_int64(0)
+        self._chk(self.lib.init_hotspots(self.ctx, float(tau0), int(nspots), _dp(tab), float(sigma), float(Tpeak),
+                                         float(alpha_floor), int(bc), C.byref(nbad)), "init_hotspots")
+        return nbad.value
+
     def finalize_ic(self, tau0, bc=_abi.FIVO_BC_OUTFLOW):
         self._chk(self.lib.finalize_ic(self.ctx, float(tau0), int(bc)), "finalize_ic")
 

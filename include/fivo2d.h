@@ -182,6 +182,15 @@ int fivo_init_from_prims(fivo_ctx* ctx, double tau0, const double* T, const doub
                          const double* ux, const double* uy, const double* Pi,
                          const double* pixx, const double* pixy, const double* piyy,
                          const double* nux, const double* nuy, int bc, int64_t* nbad);
+/* The project's seeded hot-spot event on the device: entropy-like density
+ * s = sum_k A_k exp(-|x - x_k|^2 / 2 sigma^2) over the cell centres (spots: x_k,
+ * y_k, A_k per spot, 3*nspots doubles), T = Tpeak cbrt(s / max s) floored at
+ * T_MIN, alpha = 2 log(s / max s) floored at alpha_floor, u = 0, no dissipative
+ * fields; then set_cell! on every cell and finalize_ic! as fivo_init_from_prims.
+ * The reference builds events on the host from a grid file
+ * (initialize_from_grid_csv!, main2D.jl:238-319); this builds them on the device. */
+int fivo_init_hotspots(fivo_ctx* ctx, double tau0, int32_t nspots, const double* spots, double sigma,
+                       double Tpeak, double alpha_floor, int bc, int64_t* nbad);
 /* finalize_ic! alone (main2D.jl:164-171) on the current state */
 int fivo_finalize_ic(fivo_ctx* ctx, double tau0, int bc);
 
@@ -239,8 +248,10 @@ int fivo_batch_autotune(fivo_batch* b, double tau, int32_t* tx, int32_t* ty, flo
 /* use candidate tile `tile` (0..FIVO_BATCH_NTILES-1) without timing */
 int fivo_batch_set_tile(fivo_batch* b, int32_t tile);
 /* run_sim_2d! for every event (SSPRK3, no dump callback): res[ev] per event.
- * Returns FIVO_STEP_FAILED when some event failed, like fivo_run for it alone. */
-int fivo_batch_run(fivo_batch* b, const fivo_run_opts* o, fivo_run_result* res);
+ * tau0: nullable per-event start times (else o->tau0 for all), e.g. to continue
+ * a run whose events stopped at different times. Returns FIVO_STEP_FAILED when
+ * some event failed, like fivo_run for it alone. */
+int fivo_batch_run(fivo_batch* b, const fivo_run_opts* o, const double* tau0, fivo_run_result* res);
 
 #ifdef __cplusplus
 }

@@ -102,3 +102,42 @@ def test_ensemble_autotuned_matches_oracle(oracle, hip):
         assert dT <= 1e-9 and du <= 1e-9, (k, dT, du)
         r = rel_diff(Ub, Uo)
         assert np.max(r) <= 1e-7, (k, r)
+
+
+def test_device_hotspots_match_host_generator(hip):
+    """fivo_init_hotspots builds the same event as ic.hotspot_event + set_cell!
+    (only the last bits of exp / cbrt / log may differ)."""
+    m = full_model()
+    g = F.make_grid2d(48, 40, xmax=12.0, ymax=10.0)
+    prims = ic.hotspot_event(g, seed=5, nspots=80, sigma=0.5, Tpeak=0.45)
+    a = F.Solver(g, m, lib=hip)
+    nbad_host = a.init_from_prims(0.6, **prims)
+    b = F.Solver(g, m, lib=hip)
+    nbad = b.init_hotspots(0.6, seed=5, nspots=80, sigma=0.5, Tpeak=0.45)
+    assert nbad == nbad_host          # cells left at zero by set_cell! (T at the floor)
+    r = rel_diff(b.get_state(), a.get_state())
+    print("device vs host event: max rel diff per field", np.max(r), flush=True)
+    assert np.max(r) <= 1e-12, r
+
+
+def test_ensemble_continues_from_per_event_times(hip):
+    """Two ensemble legs (the second from each event's own stop time, histories
+    kept) equal the same two legs of each event alone, bit for bit."""
+    m = full_model()
+    evs = events()
+    ens = F.Ensemble([g for g, _ in evs], m, lib=hip)
+    for s, (g, prims) in zip(ens.events, evs):
+        s.init_from_prims(RUN["tau0"], **prims)
+    leg = dict(taufinal=1e9, CFL=RUN["CFL"], CFLtau=RUN["CFLtau"])
+    r1 = ens.run_sim_2d(tau0=RUN["tau0"], max_steps=4, **leg)
+    taus = [r["tau"] for r in r1]
+    assert len(set(taus)) > 1                       # the events stopped at different times
+    r2 = ens.run_sim_2d(tau0=taus, max_steps=3, reset_history=False, **leg)
+    for k, (g, prims) in enumerate(evs):
+        s = F.Solver(g, m, lib=hip)
+        s.init_from_prims(RUN["tau0"], **prims)
+        a1 = s.run_sim_2d(tau0=RUN["tau0"], max_steps=4, integrator="ssprk3", **leg)
+        a2 = s.run_sim_2d(tau0=a1["tau"], max_steps=3, integrator="ssprk3", reset_history=False, **leg)
+        assert a1["tau"] == r1[k]["tau"] and a2["tau"] == r2[k]["tau"] and a2["nsteps"] == r2[k]["nsteps"] == 3
+        assert np.array_equal(s.get_state(), ens.events[k].get_state()), k
+    ens.close()
