@@ -751,6 +751,8 @@ int fivo_run(fivo_ctx* ctx, const fivo_run_opts* o, fivo_run_result* r) {
             r->maxu = r->dQ = r->Q1 = r->minPtot = std::numeric_limits<double>::quiet_NaN();
             r->Q0 = Qin; r->nrhs = c.nrhs; r->nhalvings = c.nhalvings; r->nmood = c.nmood;
             r->newton_iters = (int64_t)c.sc_host->newton_iters;
+            r->nlearned = (int64_t)c.sc_host->nlearned;
+            r->nlearned_fallback = (int64_t)c.sc_host->nfallback;
             return FIVO_STEP_FAILED;
         }
         tau += used;
@@ -784,6 +786,15 @@ int fivo_run(fivo_ctx* ctx, const fivo_run_opts* o, fivo_run_result* r) {
     r->Q0 = Qin; r->Q1 = Qout; r->minPtot = minp;
     r->nrhs = c.nrhs; r->nhalvings = c.nhalvings; r->nmood = c.nmood;
     r->newton_iters = (int64_t)c.sc_host->newton_iters;
+    r->nlearned = (int64_t)c.sc_host->nlearned;
+    r->nlearned_fallback = (int64_t)c.sc_host->nfallback;
+    return FIVO_OK;
+}
+
+int fivo_set_learned_seed(fivo_ctx* ctx, const fivo_mlp* m) {
+    Ctx& c = *reinterpret_cast<Ctx*>(ctx);
+    c.learned = m ? 1 : 0;
+    if (m) c.mlp = mlp_dev(m);
     return FIVO_OK;
 }
 

@@ -320,6 +320,8 @@ int fivo_batch_run(fivo_batch* bp, const fivo_run_opts* o, const double* tau0, f
         r.max_shear_res = k.max_shear; r.Q0 = Q0[e];
         r.nrhs = k.nrhs; r.nhalvings = k.nhalvings; r.nmood = 0;
         r.newton_iters = (int64_t)c.sc_host->newton_iters;
+        r.nlearned = (int64_t)c.sc_host->nlearned;
+        r.nlearned_fallback = (int64_t)c.sc_host->nfallback;
         if (k.state == EV_FAILED) {
             r.ok = 0;
             r.maxu = r.dQ = r.Q1 = r.minPtot = nan;
@@ -332,6 +334,14 @@ int fivo_batch_run(fivo_batch* bp, const fivo_run_opts* o, const double* tau0, f
         r.dQ = std::fabs(Q0[e]) > 0 ? std::fabs(Q1 - Q0[e]) / std::fabs(Q0[e]) : 0.0;
     }
     return status;
+}
+
+int fivo_batch_set_learned_seed(fivo_batch* bp, const fivo_mlp* m) {
+    Batch& b = *reinterpret_cast<Batch*>(bp);
+    b.run.learned = m ? 1 : 0;
+    if (m) b.run.mlp = mlp_dev(m);
+    for (Ctx* c : b.evs) fivo_set_learned_seed(reinterpret_cast<fivo_ctx*>(c), m);
+    return FIVO_OK;
 }
 
 }  // extern "C"

@@ -101,13 +101,35 @@ __global__ __launch_bounds__(64) void kb_coop(BatchRun R, int phase) {
     const double* U = E->U[(c->cur + phase_in_role(phase)) & 3];
     const double tau = phase_tau(c, phase);
     const PrimJob* jobs = E->jobs;
-    unsigned long long its = 0, nfail = 0, nvac = 0;
+    unsigned long long its = 0, nfail = 0, nvac = 0, nfb = 0;
     for (; t < n; t += ngroups) {
         int itsum;
-        coop_job<EK, NCH>(U, p, tau, w, jobs[t], itsum, nfail, nvac);
+        coop_job<EK, NCH>(U, p, tau, w, jobs[t], itsum, nfail, nvac, nfb);
         if (glane() == 0) its += itsum;
     }
     prim_counts(E->sc, its, nfail, nvac);
+    if (glane() == 0 && nfb) atomicAdd(&E->sc->nfallback, nfb);
+}
+
+// ---- learned seeds of the event's queued cold cells, 16 per wavefront ---------------------
+template <int EK, bool NCH>
+__global__ __launch_bounds__(64) void kb_seed(BatchRun R, int phase) {
+    const int e = block_event(R, KB_COOP);
+    CCtl c = (CCtl)(R.ctl + e);
+    if (!ev_active(c, phase)) return;
+    CEv E = (CEv)(R.ev + e);
+    const int n = (int)E->sc->ncold;
+    const int lb = blockIdx.x - E->b0[KB_COOP];
+    if (lb * 16 >= n) return;
+    const Params p = ev_params(R, E);
+    const DevWork w = ev_work(E);
+    const double* U = E->U[(c->cur + phase_in_role(phase)) & 3];
+    const double tau = phase_tau(c, phase);
+    unsigned long long used = 0;
+    for (int tile = lb; tile * 16 < n; tile += E->nb[KB_COOP])
+        used += seed_tile<EK, NCH>(U, p, tau, w, E->jobs, n, tile, R.mlp);
+    used = wave_sum(used);
+    if (threadIdx.x == 0 && used) atomicAdd(&E->sc->nlearned, used);
 }
 
 // ---- the fused face / stage-update kernel --------------------------------------------
@@ -312,6 +334,9 @@ hipError_t launch_batch_bc(Batch& b, int phase, int role) {
 hipError_t launch_batch_prim(Batch& b, int phase) {
     EOS_DISPATCH_B(b, hipLaunchKernelGGL((kb_prim<EK, NCH>), dim3(b.nblocks[KB_CELLS]), dim3(256), 0, b.stream,
                                          b.run, phase));
+    if (b.run.learned)
+        EOS_DISPATCH_B(b, hipLaunchKernelGGL((kb_seed<EK, NCH>), dim3(b.nblocks[KB_COOP]), dim3(64), 0, b.stream,
+                                             b.run, phase));
     EOS_DISPATCH_B(b, hipLaunchKernelGGL((kb_coop<EK, NCH>), dim3(b.nblocks[KB_COOP]), dim3(64), 0, b.stream,
                                          b.run, phase));
     return hipGetLastError();

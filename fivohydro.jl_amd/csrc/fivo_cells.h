@@ -17,6 +17,7 @@
 #pragma once
 #include <algorithm>
 #include "fivo_ctx.h"
+#include "fivo_mlp.h"
 
 namespace fivo {
 using namespace fd;
@@ -196,40 +197,120 @@ __device__ __forceinline__ void prim_cell(const double* __restrict__ U, const Pa
     }
 }
 
-// one queued cell, solved by the GW lanes of a group (all lanes return the same counts)
+// A learned start converges in a few iterations when the network's guess is
+// close; one that has not converged after LEARNED_ITCAP iterations is given up
+// for the cold seed, so a poor guess costs a bounded detour.
+constexpr int LEARNED_ITCAP = 12;
+
+// one queued cell, solved by the GW lanes of a group (all lanes return the same counts).
+// Job modes: it > 0 resumes the paused warm solve; it < 0 starts from the learned
+// seed left in the warm-start arrays; it == 0 starts from the cold seed. A
+// failed resumed or learned solve retries from the cold seed.
 template <int EK, bool NCH>
 __device__ __forceinline__ void coop_job(const double* __restrict__ U, const Params& p, double tau, const DevWork& w,
                                          const PrimJob& jb, int& itsum, unsigned long long& nfail,
-                                         unsigned long long& nvac) {
+                                         unsigned long long& nvac, unsigned long long& nfallback) {
     const int i = jb.cell;
     const double D = safe_div(U[i], tau), Sx = U[P_ + i], Sy = U[2 * P_ + i], E = U[3 * P_ + i];
     const CellIn c = make_cell<NCH>(D, Sx, Sy, E, load_diss(U, p, i));
     NewtonState s;
     PrimRes r;
     itsum = 0;
-    bool resume = jb.it > 0;                      // else: solve from the fresh seed
+    int mode = jb.it > 0 ? 1 : (jb.it < 0 ? 2 : 0);   // 1 resume, 2 learned seed, 0 cold seed
     const double nan = __longlong_as_double(0x7ff8000000000000LL);
 #pragma unroll 1
-    for (int pass = 0; pass < 2; ++pass) {        // a failed resumed solve retries from the seed
+    for (int pass = 0; pass < 2; ++pass) {
         bool fin = false;
-        if (resume) {
+        if (mode == 1) {
             s.x[0] = w.x0_yT[i]; s.x[1] = w.x0_phi[i]; s.x[2] = w.x0_ux[i]; s.x[3] = w.x0_uy[i];
             s.it = jb.it; s.iters = 0; s.reason = PRR_UNSET; s.converged = false;
             newton_set<EK, NCH>(s, c, p.eos);
         } else {
-            fin = newton_begin<EK, NCH, true, true>(c, nan, nan, nan, nan, p.eos, s, r);
+            const bool ls = mode == 2;
+            fin = newton_begin<EK, NCH, true, true>(c, ls ? w.x0_yT[i] : nan, ls ? w.x0_phi[i] : nan,
+                                                    ls ? w.x0_ux[i] : nan, ls ? w.x0_uy[i] : nan, p.eos, s, r);
         }
         if (!fin) {
-            if (resume && jb.it >= PRIM_JOB_TAIL) s.iters = jb.it - PRIM_JOB_TAIL;   // tail only
-            else newton_iterate<EK, NCH, true>(s, c, p.eos, NEWTON_MAXIT);
+            if (mode == 1 && jb.it >= PRIM_JOB_TAIL) s.iters = jb.it - PRIM_JOB_TAIL;   // tail only
+            else newton_iterate<EK, NCH, true>(s, c, p.eos, mode == 2 ? LEARNED_ITCAP : NEWTON_MAXIT);
+            if (mode == 2 && newton_paused(s)) {         // a learned start that is not close: give it up
+                itsum += LEARNED_ITCAP;
+                nfallback += 1;
+                mode = 0;
+                continue;
+            }
             r = newton_finish<EK, NCH, true>(s, c, p.eos);
         }
         itsum += r.iters;
-        if (!resume || r.ok || r.reason == PRR_VACUUM) break;
-        resume = false;
+        if (mode == 0 || r.ok || r.reason == PRR_VACUUM) break;
+        if (mode == 2) nfallback += 1;
+        mode = 0;
     }
     if (glane() == 0) prim_finalize(p, w, i, r, c.d, nfail, nvac);
 }
+
+// The learned seed for 16 queued cold cells (jobs [16 tile, 16 tile + 16)) on one
+// wavefront: features of the cell's conserved state -> the MLP on MFMA tiles ->
+// seed (yT, phi, ux, uy) into the warm-start arrays, job marked learned (it = -1).
+// With the charge unknown (NCH), phi is raised to at least phi_c, where the
+// scaled charge residual n u^tau / max(|D|, 1e-12) is 1e-11: in a charge-free
+// cell any phi below the tolerance floor solves the charge row, but there its
+// Jacobian row vanishes and the solve would stop; from phi_c a few capped
+// steps reach the floor while T and u converge.
+template <int EK, bool NCH>
+__device__ __forceinline__ int seed_tile(const double* __restrict__ U, const Params& p, double tau,
+                                         const DevWork& w, PrimJob* jobs, int n, int tile, const MlpDev& M) {
+    const int l = threadIdx.x & 63, col = l & 15;
+    const int j = tile * 16 + col;
+    bool cold = false;
+    int cell = 0;
+    float fn[MLP_IN] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    MlpFrame fr;
+    fr.sx = 1.0; fr.sy = 0.0;
+    if (j < n && jobs[j].it == 0) {
+        cell = jobs[j].cell;
+        const double D = safe_div(U[cell], tau), Sx = U[P_ + cell], Sy = U[2 * P_ + cell], E = U[3 * P_ + cell];
+        if (isf(E) && E > E_VAC && isf(D) && D >= 0.0 && isf(Sx) && isf(Sy)) {
+            const Diss d = load_diss(U, p, cell);
+            fr = mlp_frame(Sx, Sy);
+            double f[MLP_IN];
+            mlp_features(D, Sx, Sy, E, d, fr, f);
+            cold = true;
+            for (int k = 0; k < MLP_IN; ++k) {
+                const double z = (f[k] - (double)M.norm[k]) / (double)M.norm[6 + k];
+                cold = cold && isf(z);
+                fn[k] = isf(z) ? (float)jclamp(z, -50.0, 50.0) : 0.f;
+            }
+            if (!cold) for (int k = 0; k < MLP_IN; ++k) fn[k] = 0.f;
+        }
+    }
+    MlpFwd F;
+    mlp_forward_tile(M.tiles, M.bias, mlp_feature_operand(fn), F);
+    int used = 0;
+    if (l < 16 && cold) {
+        double x[4];
+        mlp_seed_from_outputs(F.y, M.norm, fr, x);
+        if (NCH) {
+            const double T = jmax(exp(x[0]), T_MIN);
+            const TPart tp = eos_tpart<EK>(T, p.eos);
+            double P0, n0, e0;
+            eos_assemble<EK>(tp, p.eos.m, p.eos, &P0, &n0, &e0);     // phi = 0: n = n0 e^phi
+            const double ut = sqrt(1 + x[2] * x[2] + x[3] * x[3]);
+            const double D = safe_div(U[cell], tau);
+            if (n0 > 0 && isf(n0)) {
+                const double phic = log(1e-11 * jmax(fabs(D), 1e-12) / (ut * n0));
+                if (isf(phic)) x[1] = jclamp(jmax(x[1], phic), -PHI_CAP, PHI_CAP);
+            }
+        }
+        if (isf(x[0]) && isf(x[1]) && isf(x[2]) && isf(x[3])) {
+            w.x0_yT[cell] = x[0]; w.x0_phi[cell] = x[1]; w.x0_ux[cell] = x[2]; w.x0_uy[cell] = x[3];
+            jobs[j].it = -1;
+            used = 1;
+        }
+    }
+    return used;
+}
+
 
 // ---- face physics -------------------------------------------------------------------
 struct FPrim { double T, ux, uy, n, e, P, cs2; };

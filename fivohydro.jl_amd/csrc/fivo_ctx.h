@@ -15,6 +15,7 @@
 #include <string>
 #include <vector>
 #include "fivo_device.h"
+#include "fivo_mlp.h"
 #include "../../include/fivo2d.h"
 
 namespace fivo {
@@ -50,6 +51,8 @@ struct DevScalars {
     unsigned long long shear_max_bits;
     unsigned long long npending;    // near-bound cells awaiting the inversion test
     unsigned long long ncold;       // cells of the current primitive pass queued for the cooperative solve
+    unsigned long long nlearned;    // cumulative: cells started from the learned seed
+    unsigned long long nfallback;   // cumulative: of those, cells that fell back to the cold seed
 };
 
 // Per-cell work arrays on the device (Work2D, src2d/work2d.jl:15-100)
@@ -92,6 +95,9 @@ struct Ctx {
     std::string err;
     // tile shape of the fused face kernel (chosen by grid size or autotuned)
     int tile = 0;
+    // learned warm start for cells without one (null: the reference's cold seed)
+    int learned = 0;
+    MlpDev mlp{};
     // timing
     bool timing = false;
     double t_ms[TK_N] = {0, 0, 0, 0, 0};
@@ -185,6 +191,8 @@ struct BatchRun {                  // kernel arguments shared by the ensemble ke
     int bc;
     int max_halve;
     int dissip;
+    int learned;
+    MlpDev mlp;
     double cfl, cfltau;
 };
 // phases of one replay of the step graph
@@ -221,6 +229,7 @@ hipError_t launch_batch_history(Batch& b);
 hipError_t launch_batch_relax(Batch& b);
 hipError_t launch_batch_project_shear(Batch& b);
 hipError_t launch_batch_ctl(Batch& b, int what);
+MlpDev mlp_dev(const fivo_mlp* m);
 int batch_face_blocks(const Params& p, int tile, int* tiles_y, int* nmain);
 constexpr int BATCH_NTILES = 4;    // face tile candidates of the autotuner
 void batch_tile_shape(int tile, int* tx, int* ty);

@@ -28,13 +28,26 @@ __global__ __launch_bounds__(256) void k_prim_coop(const double* __restrict__ U,
                                                    DevScalars* sc, const PrimJob* __restrict__ jobs) {
     const int n = (int)sc->ncold;
     const int ngroups = gridDim.x * (blockDim.x / GW);
-    unsigned long long its = 0, nfail = 0, nvac = 0;
+    unsigned long long its = 0, nfail = 0, nvac = 0, nfb = 0;
     for (int t = (blockIdx.x * blockDim.x + threadIdx.x) / GW; t < n; t += ngroups) {
         int itsum;
-        coop_job<EK, NCH>(U, p, tau, w, jobs[t], itsum, nfail, nvac);
+        coop_job<EK, NCH>(U, p, tau, w, jobs[t], itsum, nfail, nvac, nfb);
         if (glane() == 0) its += itsum;
     }
     prim_counts(sc, its, nfail, nvac);
+    if (glane() == 0 && nfb) atomicAdd(&sc->nfallback, nfb);
+}
+
+// learned seeds for the queued cold cells, one wavefront per 16 cells
+template <int EK, bool NCH>
+__global__ __launch_bounds__(64) void k_prim_seed(const double* __restrict__ U, Params p, double tau, DevWork w,
+                                                  DevScalars* sc, PrimJob* jobs, MlpDev M) {
+    const int n = (int)sc->ncold;
+    unsigned long long used = 0;
+    for (int tile = blockIdx.x; tile * 16 < n; tile += gridDim.x)
+        used += seed_tile<EK, NCH>(U, p, tau, w, jobs, n, tile, M);
+    used = wave_sum(used);
+    if (threadIdx.x == 0 && used) atomicAdd(&sc->nlearned, used);
 }
 
 template <int EK, bool NCH, int TX, int TY, int MODE>
@@ -226,6 +239,9 @@ hipError_t launch_prim(Ctx& c, const double* U, double tau) {
     // the worklist counter is cleared with the other per-pass scalars
     EOS_DISPATCH(c, hipLaunchKernelGGL((k_prim<EK, NCH>), dim3(nblk(p.Ntot, 256)), dim3(256), 0, c.stream, U, p,
                                        tau, c.w, c.sc, c.jobs));
+    if (c.learned)
+        EOS_DISPATCH(c, hipLaunchKernelGGL((k_prim_seed<EK, NCH>), dim3(std::min(nblk(p.Ntot, 16), 4096)), dim3(64), 0,
+                                           c.stream, U, p, tau, c.w, c.sc, c.jobs, c.mlp));
     // one group of GW lanes per queued cell; 2048 blocks hold 64k groups, more
     // than the queue of a warm pass, and stride over the queue of a cold one
     const int nb = std::min(nblk((int64_t)p.Ntot * GW, 256), 2048);

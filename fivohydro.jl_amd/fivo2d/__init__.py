@@ -11,9 +11,10 @@ from .model import (ConformalHQEOS, LatticeHRGEOS, Grid2D, make_grid2d, make_gri
                     HBARC)
 from .solver import Solver, FivoError
 from .batch import Ensemble
+from .mlp import Mlp
 from . import ic
 
 __all__ = ["FIVO_BC_OUTFLOW", "FIVO_BC_PERIODIC", "FIVO_SSPRK2", "FIVO_SSPRK3", "Lib", "load_hip",
            "HIP_LIB_PATH", "ConformalHQEOS", "LatticeHRGEOS", "Grid2D", "make_grid2d",
            "make_grid_2d", "make_layout2d", "build_model_2d", "allocate_state", "resolve_terms",
-           "Solver", "FivoError", "Ensemble", "ic", "T_MIN", "E_FLOOR", "HBARC"]
+           "Solver", "FivoError", "Ensemble", "Mlp", "ic", "T_MIN", "E_FLOOR", "HBARC"]

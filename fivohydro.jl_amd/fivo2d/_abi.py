@@ -64,6 +64,7 @@ class ModelDesc(C.Structure):
 
 
 _DP = C.POINTER(C.c_double)
+_FP = C.POINTER(C.c_float)
 
 
 class WorkView(C.Structure):
@@ -100,7 +101,7 @@ class RunResult(C.Structure):
                 ("max_shear_res", C.c_double), ("maxu", C.c_double), ("dQ", C.c_double),
                 ("Q0", C.c_double), ("Q1", C.c_double), ("minPtot", C.c_double),
                 ("nrhs", C.c_int64), ("nhalvings", C.c_int64), ("nmood", C.c_int64),
-                ("newton_iters", C.c_int64)]
+                ("newton_iters", C.c_int64), ("nlearned", C.c_int64), ("nlearned_fallback", C.c_int64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_ if not f.startswith("_")}
@@ -183,7 +184,21 @@ def load_hip():
                                 ("batch_set_tile", C.c_int, [C.c_void_p, C.c_int32]),
                                 ("batch_run", C.c_int, [C.c_void_p, C.POINTER(RunOpts), _DP, C.POINTER(RunResult)]),
                                 ("init_hotspots", C.c_int, [C.c_void_p, C.c_double, C.c_int32, _DP, C.c_double,
-                                                            C.c_double, C.c_double, C.c_int, C.POINTER(C.c_int64)])):
+                                                            C.c_double, C.c_double, C.c_int, C.POINTER(C.c_int64)]),
+                                ("mlp_create", C.c_int, [C.c_int64, C.c_uint64, C.POINTER(C.c_void_p)]),
+                                ("mlp_destroy", None, [C.c_void_p]),
+                                ("mlp_last_error", C.c_char_p, [C.c_void_p]),
+                                ("mlp_collect", C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.POINTER(C.c_int64)]),
+                                ("mlp_size", C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
+                                ("mlp_fit_normalization", C.c_int, [C.c_void_p]),
+                                ("mlp_train", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_float, _FP]),
+                                ("mlp_get_params", C.c_int, [C.c_void_p, _FP, _FP]),
+                                ("mlp_set_params", C.c_int, [C.c_void_p, _FP, _FP]),
+                                ("mlp_forward", C.c_int, [C.c_void_p, C.c_int64, _FP, _FP]),
+                                ("mlp_eval", C.c_int, [C.c_void_p, _FP]),
+                                ("mlp_grad", C.c_int, [C.c_void_p, C.c_int64, _FP, _FP, _FP, _FP]),
+                                ("set_learned_seed", C.c_int, [C.c_void_p, C.c_void_p]),
+                                ("batch_set_learned_seed", C.c_int, [C.c_void_p, C.c_void_p])):
             f = getattr(lib.dll, "fivo_" + name)
             f.restype = res
             f.argtypes = args

@@ -63,6 +63,10 @@ class Ensemble:
         self.tile = (tx.value, ty.value)
         return self.tile, {_abi.BATCH_TILES[k]: ms[k] for k in range(_abi.FIVO_BATCH_NTILES)}
 
+    def set_learned_seed(self, mlp):
+        """Start every event's cells without a warm start from `mlp`'s guess (None: the cold seed)."""
+        self._chk(self.lib.batch_set_learned_seed(self.b, None if mlp is None else mlp.m), "batch_set_learned_seed")
+
     def set_tile(self, tile):
         k = _abi.BATCH_TILES.index(tuple(tile))
         self._chk(self.lib.batch_set_tile(self.b, k), "batch_set_tile")

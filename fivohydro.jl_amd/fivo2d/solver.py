@@ -151,6 +151,10 @@ class Solver:
                                          float(alpha_floor), int(bc), C.byref(nbad)), "init_hotspots")
         return nbad.value
 
+    def set_learned_seed(self, mlp):
+        """Start cells without a warm start from `mlp`'s guess (None: the cold seed)."""
+        self._chk(self.lib.set_learned_seed(self.ctx, None if mlp is None else mlp.m), "set_learned_seed")
+
     def finalize_ic(self, tau0, bc=_abi.FIVO_BC_OUTFLOW):
         self._chk(self.lib.finalize_ic(self.ctx, float(tau0), int(bc)), "finalize_ic")
 

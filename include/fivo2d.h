@@ -152,6 +152,8 @@ typedef struct fivo_run_result {
     /* extra accounting, not in the reference NamedTuple */
     int64_t nrhs, nhalvings, nmood;
     int64_t newton_iters;
+    int64_t nlearned;            /* cells started from the learned seed */
+    int64_t nlearned_fallback;   /* of those, cells that fell back to the cold seed */
 } fivo_run_result;
 
 typedef struct fivo_ctx fivo_ctx;
@@ -226,6 +228,45 @@ int fivo_run(fivo_ctx* ctx, const fivo_run_opts* o, fivo_run_result* r);
 int64_t fivo_kernel_time(const fivo_ctx* ctx, int category, double* total_ms);
 int fivo_enable_timing(fivo_ctx* ctx, int on);
 
+/* ---- learned Newton warm start --------------------------------------------- */
+/* A 6 -> 64 -> 64 -> 6 GELU MLP (BF16 MFMA, FP32 accumulate) mapping features of a
+ * cell's conserved state (log E, log(D + 1e-30), |S|/E, Pi/E, s.pi.s/E,
+ * s.nu/(|nu| + D), s = S/|S|) to its primitives (yT, phi, asinh(u.s), u.s_perp,
+ * log e, log P). With a context's learned seed set, cells without a warm start
+ * start Newton from the network's output instead of seed_from_conserved_2d
+ * (src2d/primrec2d.jl:323-347); Newton still converges to the reference
+ * tolerance, and a cell whose solve fails falls back to the cold seed. No
+ * counterpart in the reference. Parameters: W1[64][6], b1[64], W2[64][64],
+ * b2[64], W3[6][64], b3[6] (FIVO_MLP_NPARAM floats); normalisation: feature
+ * mean[6], std[6], target mean[6], std[6]. */
+#define FIVO_MLP_NPARAM 4998
+#define FIVO_MLP_NNORM 24
+typedef struct fivo_mlp fivo_mlp;
+/* an untrained network (Glorot-uniform from `seed`) with room for `capacity` pairs */
+int fivo_mlp_create(int64_t capacity, uint64_t seed, fivo_mlp** out);
+void fivo_mlp_destroy(fivo_mlp* m);
+const char* fivo_mlp_last_error(const fivo_mlp* m);
+/* append the (features, targets) pairs of the converged interior cells of the
+ * context's last primitive pass at tau; total = pairs held */
+int fivo_mlp_collect(fivo_mlp* m, fivo_ctx* ctx, double tau, int64_t* total);
+int fivo_mlp_size(const fivo_mlp* m, int64_t* npairs);
+/* mean / std of the held pairs; normalises them in place (collection ends) */
+int fivo_mlp_fit_normalization(fivo_mlp* m);
+/* Adam on minibatches of the held pairs; loss[epoch] = mean over pairs of the
+ * summed squared error of the normalised outputs during that epoch */
+int fivo_mlp_train(fivo_mlp* m, int32_t epochs, int32_t batch, float lr, float* loss);
+int fivo_mlp_get_params(const fivo_mlp* m, float* params, float* norm);   /* either nullable */
+int fivo_mlp_set_params(fivo_mlp* m, const float* params, const float* norm);
+/* mean squared error of each normalised output over the held pairs */
+int fivo_mlp_eval(fivo_mlp* m, float* mse);
+/* the network on n normalised feature rows (n x 6) -> normalised outputs (n x 6) */
+int fivo_mlp_forward(fivo_mlp* m, int64_t n, const float* x, float* y);
+/* loss and its gradient w.r.t. the parameters on n normalised (x, target) rows */
+int fivo_mlp_grad(fivo_mlp* m, int64_t n, const float* x, const float* t, float* loss, float* grad);
+/* use the network for the context's cold starts (m = NULL: the cold seed);
+ * the network must outlive its use */
+int fivo_set_learned_seed(fivo_ctx* ctx, const fivo_mlp* m);
+
 /* ---- ensembles of events ------------------------------------------------- */
 /* The reference runs fluctuating events one at a time, each with its own
  * run_sim_2d! (main2D.jl:371-561). An ensemble runs many such events at once:
@@ -252,6 +293,8 @@ int fivo_batch_set_tile(fivo_batch* b, int32_t tile);
  * a run whose events stopped at different times. Returns FIVO_STEP_FAILED when
  * some event failed, like fivo_run for it alone. */
 int fivo_batch_run(fivo_batch* b, const fivo_run_opts* o, const double* tau0, fivo_run_result* res);
+/* fivo_set_learned_seed for every event of the ensemble */
+int fivo_batch_set_learned_seed(fivo_batch* b, const fivo_mlp* m);
 
 #ifdef __cplusplus
 }
