@@ -29,6 +29,16 @@ import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per launch of the dominant kernel measured with rocprofv3 --pmc
+# FETCH_SIZE / WRITE_SIZE, scaled by the same counters on a device copy of known
+# size (profiles/full_sectors_512_pmc_traffic.md); keyed by (config, kernel).
+TRAFFIC = {
+    ("c3", "k_prim_coop"): (2.0 * 1010.4 + 742.3) * 1024,
+    ("c3", "k_prim"): (2.0 * 15255.2 + 33697.1) * 1024,
+    ("c3", "k_faces"): (2.0 * 41842.8 + 34183.4) * 1024,
+    ("c3", "k_relax"): (2.0 * 36540.1 + 56558.3) * 1024,
+}
+
 # Workloads: (description, builder). Algorithmic HBM bytes per cell-update and per
 # dominant-kernel launch follow the per-array model: every array of a pass is read
 # or written once per interior cell.
@@ -211,6 +221,26 @@ def faces_bytes_per_launch(cfg, ncell):
     return 8 * nv * (2 + 3 * (s - 1)) / s * ncell
 
 
+def kernel_alg_bytes(kernel, cfg, g, r, launches):
+    """Algorithmic HBM bytes of one launch of `kernel`, from the per-array model:
+    faces: the stage's RK state (2 nv doubles in stage 1, 3 nv in the weighted
+    stages) per interior cell; warm primitive kernel: the Newton warm starts, 8
+    doubles per cell; cooperative kernel: per queued cell the state (nv) and warm
+    start (4) read, primitives (11) and warm start (4) written; relaxation: U read,
+    dissipative rows written, histories read and written."""
+    ncell = g.Nx * g.Ny
+    nv = cfg["nv"]
+    if kernel == "k_faces":
+        return faces_bytes_per_launch(cfg, ncell)
+    if kernel == "k_prim":
+        return 8 * 8 * g.Ntot
+    if kernel == "k_prim_coop":
+        return 8 * (nv + 19) * r["nqueued"] / max(launches, 1)
+    if kernel == "k_relax":
+        return 8 * (nv + (nv - 4) + 12) * ncell
+    return 8 * nv * 2 * ncell
+
+
 def make_problem(cfg, lib):
     import fivo2d as F
     from fivo2d import ic
@@ -268,6 +298,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--ensemble-steps", type=int, default=10)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a device copy of 512 MiB (the reference kernel of the traffic counters)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
@@ -289,23 +321,18 @@ def main():
     wall = time.perf_counter() - t0
     r_plain = r
     value = a.steps * ncell / wall
-    # dominant kernel from the event timings of the timed region
+    # dominant kernel from the event timings of the timed region (HIP events on
+    # the library's stream, one pair around each launch)
     tms = {}
-    for cat, name in ((0, "k_prim"), (1, "k_faces"), (2, "k_bc"), (3, "k_relax")):
+    for cat, name in ((0, "k_prim"), (5, "k_prim_coop"), (6, "k_prim_seed"), (1, "k_faces"), (2, "k_bc"),
+                      (3, "k_relax")):
         tot = C.c_double(0)
         cnt = lib.kernel_time(s.ctx, cat, C.byref(tot))
         if cnt > 0:
             tms[name] = (tot.value, cnt)
     dom = max(tms, key=lambda k: tms[k][0])
     avg_ms = tms[dom][0] / tms[dom][1]
-    if dom == "k_faces":
-        alg = faces_bytes_per_launch(cfg, ncell)
-    elif dom == "k_prim":
-        alg = 8 * 8 * g.Ntot          # Newton warm starts, 8 doubles per cell per pass
-    elif dom == "k_relax":
-        alg = 8 * (cfg["nv"] + (cfg["nv"] - 4) + 12) * ncell
-    else:
-        alg = 8 * cfg["nv"] * 2 * ncell
+    alg = kernel_alg_bytes(dom, cfg, g, r, tms[dom][1])
     achieved = alg / (avg_ms * 1e-3) / 1e9
     out = {
         "metric": "cell-updates/s, plain vs batched+learned, 512² viscous+charm, ΔT≤1e-9",
@@ -318,9 +345,19 @@ def main():
                    "kernel_ms": {k: round(v[0] / v[1], 5) for k, v in tms.items()},
                    "kernel_launches": {k: v[1] for k, v in tms.items()}},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC.get((a.config, dom)),
+                     "alg_bytes_per_launch": alg, "avg_launch_ms": avg_ms,
                      "step_achieved_GBs": algorithmic_bytes(cfg) * value / 1e9},
     }
+    if a.copy_calibration:
+        src = torch.empty(64 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        src.fill_(1.0)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        out["copy_calibration_bytes"] = src.numel() * 8
     if not a.no_ensemble:
         net, tinfo = train_warm_start(lib)
         out["learned_warm_start"] = tinfo

@@ -126,8 +126,16 @@ int do_bc(Ctx& c, double* U, int bc) {
     return FIVO_OK;
 }
 int do_prim(Ctx& c, const double* U, double tau) {
-    Timer t(c, TK_PRIM);
-    HIPCHK(&c, launch_prim(c, U, tau));
+    {
+        Timer t(c, TK_PRIM);
+        HIPCHK(&c, launch_prim_warm(c, U, tau));
+    }
+    if (c.learned) {
+        Timer t(c, TK_SEED);
+        HIPCHK(&c, launch_prim_seed(c, U, tau));
+    }
+    Timer t(c, TK_COOP);
+    HIPCHK(&c, launch_prim_coop(c, U, tau));
     return FIVO_OK;
 }
 int do_faces(Ctx& c, const FaceLaunch& f) {
@@ -753,6 +761,7 @@ int fivo_run(fivo_ctx* ctx, const fivo_run_opts* o, fivo_run_result* r) {
             r->newton_iters = (int64_t)c.sc_host->newton_iters;
             r->nlearned = (int64_t)c.sc_host->nlearned;
             r->nlearned_fallback = (int64_t)c.sc_host->nfallback;
+            r->nqueued = (int64_t)c.sc_host->nqueued;
             return FIVO_STEP_FAILED;
         }
         tau += used;
@@ -788,6 +797,7 @@ int fivo_run(fivo_ctx* ctx, const fivo_run_opts* o, fivo_run_result* r) {
     r->newton_iters = (int64_t)c.sc_host->newton_iters;
     r->nlearned = (int64_t)c.sc_host->nlearned;
     r->nlearned_fallback = (int64_t)c.sc_host->nfallback;
+    r->nqueued = (int64_t)c.sc_host->nqueued;
     return FIVO_OK;
 }
 

@@ -322,6 +322,7 @@ int fivo_batch_run(fivo_batch* bp, const fivo_run_opts* o, const double* tau0, f
         r.newton_iters = (int64_t)c.sc_host->newton_iters;
         r.nlearned = (int64_t)c.sc_host->nlearned;
         r.nlearned_fallback = (int64_t)c.sc_host->nfallback;
+        r.nqueued = (int64_t)c.sc_host->nqueued;
         if (k.state == EV_FAILED) {
             r.ok = 0;
             r.maxu = r.dQ = r.Q1 = r.minPtot = nan;

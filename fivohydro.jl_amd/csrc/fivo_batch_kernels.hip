@@ -79,9 +79,9 @@ __global__ __launch_bounds__(256) void kb_prim(BatchRun R, int phase) {
     const double* U = E->U[(c->cur + phase_in_role(phase)) & 3];
     const double tau = phase_tau(c, phase);
     const int i = (blockIdx.x - E->b0[KB_CELLS]) * 256 + threadIdx.x;
-    unsigned long long its = 0, nfail = 0, nvac = 0;
-    if (i < p.Ntot) prim_cell<EK, NCH>(U, p, tau, w, i, e, &E->sc->ncold, E->jobs, its, nfail, nvac);
-    prim_counts(E->sc, its, nfail, nvac);
+    unsigned long long its = 0, nfail = 0, nvac = 0, nq = 0;
+    if (i < p.Ntot) prim_cell<EK, NCH>(U, p, tau, w, i, e, &E->sc->ncold, E->jobs, its, nfail, nvac, nq);
+    prim_counts(E->sc, its, nfail, nvac, nq);
 }
 
 // ---- primitive pass: the event's queued cells, GW lanes per cell ------------------------

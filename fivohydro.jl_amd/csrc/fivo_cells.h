@@ -145,12 +145,13 @@ __device__ __forceinline__ void prim_finalize(const Params& p, const DevWork& w,
 }
 
 __device__ __forceinline__ void prim_counts(DevScalars* sc, unsigned long long its, unsigned long long nfail,
-                                            unsigned long long nvac) {
-    its = wave_sum(its); nfail = wave_sum(nfail); nvac = wave_sum(nvac);
+                                            unsigned long long nvac, unsigned long long nq = 0) {
+    its = wave_sum(its); nfail = wave_sum(nfail); nvac = wave_sum(nvac); nq = wave_sum(nq);
     if ((threadIdx.x & 63) == 0) {
         if (its) atomicAdd(&sc->newton_iters, its);
         if (nfail) atomicAdd(&sc->nprimfail, nfail);
         if (nvac) atomicAdd(&sc->nvacuum, nvac);
+        if (nq) atomicAdd(&sc->nqueued, nq);
     }
 }
 
@@ -159,7 +160,7 @@ template <int EK, bool NCH>
 __device__ __forceinline__ void prim_cell(const double* __restrict__ U, const Params& p, double tau,
                                           const DevWork& w, int i, int ev, unsigned long long* njobs,
                                           PrimJob* jobs, unsigned long long& its, unsigned long long& nfail,
-                                          unsigned long long& nvac) {
+                                          unsigned long long& nvac, unsigned long long& nq) {
     const double D = safe_div(U[i], tau), Sx = U[P_ + i], Sy = U[2 * P_ + i], E = U[3 * P_ + i];
     const CellIn c = make_cell<NCH>(D, Sx, Sy, E, load_diss(U, p, i));
     const double yT0 = w.x0_yT[i], ux0 = w.x0_ux[i], uy0 = w.x0_uy[i];
@@ -194,6 +195,7 @@ __device__ __forceinline__ void prim_cell(const double* __restrict__ U, const Pa
     if (job >= 0) {
         const unsigned long long k = atomicAdd(njobs, 1ull);
         jobs[k] = PrimJob{i, job, ev};
+        nq += 1;
     }
 }
 

@@ -53,6 +53,7 @@ struct DevScalars {
     unsigned long long ncold;       // cells of the current primitive pass queued for the cooperative solve
     unsigned long long nlearned;    // cumulative: cells started from the learned seed
     unsigned long long nfallback;   // cumulative: of those, cells that fell back to the cold seed
+    unsigned long long nqueued;     // cumulative: cells queued for the cooperative kernel
 };
 
 // Per-cell work arrays on the device (Work2D, src2d/work2d.jl:15-100)
@@ -71,7 +72,9 @@ struct DevWork {
     double *gradAx, *gradAy;
 };
 
-enum { TK_PRIM = 0, TK_FACES = 1, TK_BC = 2, TK_RELAX = 3, TK_ALL = 4, TK_N = 5 };
+// timing categories: warm primitive kernel, fused faces, ghost fill, relaxation,
+// everything, cooperative primitive kernel, learned seeds
+enum { TK_PRIM = 0, TK_FACES = 1, TK_BC = 2, TK_RELAX = 3, TK_ALL = 4, TK_COOP = 5, TK_SEED = 6, TK_N = 7 };
 
 struct Ctx {
     fivo_grid_desc gd;
@@ -100,8 +103,8 @@ struct Ctx {
     MlpDev mlp{};
     // timing
     bool timing = false;
-    double t_ms[TK_N] = {0, 0, 0, 0, 0};
-    int64_t t_cnt[TK_N] = {0, 0, 0, 0, 0};
+    double t_ms[TK_N] = {0, 0, 0, 0, 0, 0, 0};
+    int64_t t_cnt[TK_N] = {0, 0, 0, 0, 0, 0, 0};
     hipEvent_t ev[2] = {nullptr, nullptr};
     std::vector<hipEvent_t> evpool;
     std::vector<int> ev_cat;
@@ -123,7 +126,10 @@ struct FaceLaunch {
     const uint8_t* force;  // MOOD first-order mask or null
 };
 hipError_t launch_bc(Ctx& c, double* U, int bc);
-hipError_t launch_prim(Ctx& c, const double* U, double tau);
+hipError_t launch_prim(Ctx& c, const double* U, double tau);        // the three below in order
+hipError_t launch_prim_warm(Ctx& c, const double* U, double tau);
+hipError_t launch_prim_seed(Ctx& c, const double* U, double tau);   // no-op without a learned seed
+hipError_t launch_prim_coop(Ctx& c, const double* U, double tau);
 hipError_t launch_faces(Ctx& c, const FaceLaunch& f);
 hipError_t launch_expand(Ctx& c, const uint8_t* in, uint8_t* out);
 hipError_t launch_sanitize(Ctx& c, double* U, double tau, int use_P);

@@ -18,9 +18,9 @@ template <int EK, bool NCH>
 __global__ __launch_bounds__(256) void k_prim(const double* __restrict__ U, Params p, double tau, DevWork w,
                                               DevScalars* sc, PrimJob* jobs) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    unsigned long long its = 0, nfail = 0, nvac = 0;
-    if (i < p.Ntot) prim_cell<EK, NCH>(U, p, tau, w, i, 0, &sc->ncold, jobs, its, nfail, nvac);
-    prim_counts(sc, its, nfail, nvac);
+    unsigned long long its = 0, nfail = 0, nvac = 0, nq = 0;
+    if (i < p.Ntot) prim_cell<EK, NCH>(U, p, tau, w, i, 0, &sc->ncold, jobs, its, nfail, nvac, nq);
+    prim_counts(sc, its, nfail, nvac, nq);
 }
 
 template <int EK, bool NCH>
@@ -234,20 +234,34 @@ hipError_t launch_bc(Ctx& c, double* U, int bc) {
     return hipGetLastError();
 }
 
-hipError_t launch_prim(Ctx& c, const double* U, double tau) {
+hipError_t launch_prim_warm(Ctx& c, const double* U, double tau) {
     const Params& p = c.p;
     // the worklist counter is cleared with the other per-pass scalars
     EOS_DISPATCH(c, hipLaunchKernelGGL((k_prim<EK, NCH>), dim3(nblk(p.Ntot, 256)), dim3(256), 0, c.stream, U, p,
                                        tau, c.w, c.sc, c.jobs));
-    if (c.learned)
-        EOS_DISPATCH(c, hipLaunchKernelGGL((k_prim_seed<EK, NCH>), dim3(std::min(nblk(p.Ntot, 16), 4096)), dim3(64), 0,
-                                           c.stream, U, p, tau, c.w, c.sc, c.jobs, c.mlp));
+    return hipGetLastError();
+}
+hipError_t launch_prim_seed(Ctx& c, const double* U, double tau) {
+    if (!c.learned) return hipSuccess;
+    const Params& p = c.p;
+    EOS_DISPATCH(c, hipLaunchKernelGGL((k_prim_seed<EK, NCH>), dim3(std::min(nblk(p.Ntot, 16), 4096)), dim3(64), 0,
+                                       c.stream, U, p, tau, c.w, c.sc, c.jobs, c.mlp));
+    return hipGetLastError();
+}
+hipError_t launch_prim_coop(Ctx& c, const double* U, double tau) {
+    const Params& p = c.p;
     // one group of GW lanes per queued cell; 2048 blocks hold 64k groups, more
     // than the queue of a warm pass, and stride over the queue of a cold one
     const int nb = std::min(nblk((int64_t)p.Ntot * GW, 256), 2048);
     EOS_DISPATCH(c, hipLaunchKernelGGL((k_prim_coop<EK, NCH>), dim3(nb), dim3(256), 0, c.stream, U, p, tau, c.w,
                                        c.sc, c.jobs));
     return hipGetLastError();
+}
+hipError_t launch_prim(Ctx& c, const double* U, double tau) {
+    hipError_t e = launch_prim_warm(c, U, tau);
+    if (e == hipSuccess) e = launch_prim_seed(c, U, tau);
+    if (e == hipSuccess) e = launch_prim_coop(c, U, tau);
+    return e;
 }
 
 template <int EK, bool NCH, int TX, int TY>

@@ -101,7 +101,8 @@ class RunResult(C.Structure):
                 ("max_shear_res", C.c_double), ("maxu", C.c_double), ("dQ", C.c_double),
                 ("Q0", C.c_double), ("Q1", C.c_double), ("minPtot", C.c_double),
                 ("nrhs", C.c_int64), ("nhalvings", C.c_int64), ("nmood", C.c_int64),
-                ("newton_iters", C.c_int64), ("nlearned", C.c_int64), ("nlearned_fallback", C.c_int64)]
+                ("newton_iters", C.c_int64), ("nlearned", C.c_int64), ("nlearned_fallback", C.c_int64),
+                ("nqueued", C.c_int64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_ if not f.startswith("_")}

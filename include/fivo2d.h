@@ -154,6 +154,7 @@ typedef struct fivo_run_result {
     int64_t newton_iters;
     int64_t nlearned;            /* cells started from the learned seed */
     int64_t nlearned_fallback;   /* of those, cells that fell back to the cold seed */
+    int64_t nqueued;             /* cells handed from the warm to the cooperative primitive kernel */
 } fivo_run_result;
 
 typedef struct fivo_ctx fivo_ctx;
@@ -222,9 +223,11 @@ int fivo_project_shear(fivo_ctx* ctx, double* max_rel, double* mean_rel);
 int fivo_run(fivo_ctx* ctx, const fivo_run_opts* o, fivo_run_result* r);
 
 /* ---- timing hooks for benchmarks --------------------------------------- */
-/* Average device time (ms) of the last fivo_run's kernels by category, measured
- * with HIP events on the context stream: 0 = primitive pass, 1 = faces+update,
- * 2 = boundary fill, 3 = relaxation, 4 = everything. Returns count of launches. */
+/* Total device time (ms) of the kernels by category since fivo_enable_timing,
+ * measured with HIP events on the context stream: 0 = primitive pass (warm
+ * kernel), 1 = faces+update, 2 = boundary fill, 3 = relaxation, 4 = everything,
+ * 5 = primitive pass (cooperative kernel), 6 = learned seeds. Returns the count
+ * of launches. */
 int64_t fivo_kernel_time(const fivo_ctx* ctx, int category, double* total_ms);
 int fivo_enable_timing(fivo_ctx* ctx, int on);
 
