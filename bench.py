@@ -29,6 +29,11 @@ import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 
+
+def log(msg):
+    """Progress of the legs on stderr; the result is the one JSON line on stdout."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
 # HBM bytes per launch of the dominant kernel measured with rocprofv3 --pmc
 # FETCH_SIZE / WRITE_SIZE, scaled by the same counters on a device copy of known
 # size (profiles/full_sectors_512_pmc_traffic.md); keyed by (config, kernel).
@@ -130,6 +135,7 @@ def train_warm_start(lib):
     for epochs, lr in ((15, 3e-3), (10, 1e-3), (5, 3e-4)):
         losses += list(net.train(epochs=epochs, batch=4096, lr=lr))
     t_train = time.perf_counter() - t0
+    log(f"warm start: {net.npairs} pairs in {t_collect:.1f} s, trained in {t_train:.1f} s")
     mse = net.eval()
     info = {"pairs": net.npairs, "collect_s": round(t_collect, 3), "train_s": round(t_train, 3),
             "epochs": len(losses), "loss_first_last": [float(losses[0]), float(losses[-1])],
@@ -199,6 +205,7 @@ def ensemble_bench(lib, name, steps, warmup, net):
             info["learned_starts"] = nl
             info["fallback_rate"] = sum(r["nlearned_fallback"] for r in res) / max(nl, 1)
         out[path] = info
+        log(f"{name} {path}: {info['cell_updates_per_s']:.3e} cell-updates/s, max dT/T {dT:.1e}")
         x.close()
     out["speedup_batched_vs_plain"] = out["plain"]["ms_per_step_all_events"] / out["batched"]["ms_per_step_all_events"]
     out["speedup_batched_learned_vs_plain"] = (out["plain"]["ms_per_step_all_events"] /
@@ -241,7 +248,7 @@ def kernel_alg_bytes(kernel, cfg, g, r, launches):
     return 8 * nv * 2 * ncell
 
 
-def make_problem(cfg, lib):
+def make_inputs(cfg):
     import fivo2d as F
     from fivo2d import ic
     N, L = cfg["N"], cfg["L"]
@@ -254,9 +261,45 @@ def make_problem(cfg, lib):
                              enable_bulk=True, zeta_over_s=0.1, tauPi_coeff=15.0, enable_diff=True,
                              kappa_coeff=0.1163, pi_clip_factor=1.0, consistent_m2=cfg["m2"])
         prims = ic.hotspot_event(g, seed=12345, nspots=cfg["nspots"], sigma=cfg["sigma"], Tpeak=0.45)
+    return g, m, prims
+
+
+def make_problem(cfg, lib):
+    import fivo2d as F
+    g, m, prims = make_inputs(cfg)
     s = F.Solver(g, m, lib=lib)
     s.init_from_prims(cfg["tau0"], **prims)
     return g, m, s
+
+
+def batched_learned_single(cfg, lib, net, steps, warmup, T_plain, wall_plain):
+    """The workload's one event through the ensemble path (every phase one launch,
+    graph-replayed SSPRK3 steps, autotuned face tile) with the learned warm start;
+    same steps as the plain run, final T against it."""
+    import torch
+    import fivo2d as F
+    g, m, prims = make_inputs(cfg)
+    x = F.Ensemble([g], m, lib=lib)
+    x.events[0].init_from_prims(cfg["tau0"], **prims)
+    x.set_learned_seed(net)
+    tile, _ = x.autotune(cfg["tau0"])
+    leg = dict(taufinal=1e9, CFL=cfg["cfl"], CFLtau=cfg["cfltau"])
+    r0 = x.run_sim_2d(tau0=cfg["tau0"], max_steps=warmup, **leg)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = x.run_sim_2d(tau0=[r0[0]["tau"]], max_steps=steps, reset_history=False, **leg)[0]
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    assert r["ok"] and r["nsteps"] == steps, r
+    T = np.exp(x.events[0].get_work(("yT",))["yT"])
+    ncell = g.Nx * g.Ny
+    out = {"cell_updates_per_s": steps * ncell / wall, "ms_per_step": wall / steps * 1e3,
+           "speedup_vs_plain": wall_plain / wall, "tile": list(tile),
+           "max_dT_over_T_vs_plain": float(np.max(np.abs(T - T_plain) / T_plain)),
+           "learned_starts": r["nlearned"], "fallback_rate": r["nlearned_fallback"] / max(r["nlearned"], 1),
+           "newton_iters_per_cell_update": r["newton_iters"] / (ncell * steps)}
+    x.close()
+    return out
 
 
 def run_steps(s, tau, nsteps, cfg, reset):
@@ -321,6 +364,7 @@ def main():
     wall = time.perf_counter() - t0
     r_plain = r
     value = a.steps * ncell / wall
+    log(f"{cfg['workload']} plain: {value:.3e} cell-updates/s")
     # dominant kernel from the event timings of the timed region (HIP events on
     # the library's stream, one pair around each launch)
     tms = {}
@@ -380,11 +424,15 @@ def main():
                 "newton_iters_per_cell_update": r["newton_iters"] / (ncell * a.steps),
                 "newton_iters_per_cell_update_plain": r_plain["newton_iters"] / (ncell * a.steps)}
             s2.close()
+            log(f"{cfg['workload']} plain+learned: {out['plain_learned_512']['cell_updates_per_s']:.3e}")
+            out["batched_learned_512"] = batched_learned_single(cfg, lib, net, a.steps, max(a.warmup, 1), Tp, wall)
+            log(f"{cfg['workload']} batched+learned: {out['batched_learned_512']['cell_updates_per_s']:.3e}")
         out["ensemble_c4"] = ensemble_bench(lib, "c4", a.ensemble_steps, 3, net)
         out["ensemble_c5"] = ensemble_bench(lib, "c5", a.ensemble_steps, 3, net)
         net.close()
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cfg)
+        log(f"cpu baseline: {out['cpu_baseline']['value']:.3e} cell-updates/s")
     print(json.dumps(out))
 
 

@@ -1,0 +1,114 @@
+"""The reference's validation gates run on the HIP path.
+
+Each gate is run twice, by the HIP library and by the oracle, on the same
+inputs: the HIP run must pass the gate's own bounds (the published numbers
+and thresholds of test/test_gubser2d.jl, test/test_bjorken2d.jl,
+test/test_bjorken_bulk2d.jl, test/test_dissipation2d.jl, test/test_charge2d.jl)
+and land on the oracle's final state within the final-time bound (1e-9 relative
+in T, u and the dissipative fields' scale).
+"""
+import numpy as np
+import pytest
+
+import fivo2d as F
+import test_oracle_dissipative_gates as D
+import test_oracle_gates as G
+from oracle_unit import OracleUnit
+
+pytestmark = pytest.mark.gpu
+
+
+def test_gubser_gate_g1_on_gpu(oracle, hip):
+    for N, (eT, eU) in ((100, (3.29e-3, 7.73e-3)), (200, (8.81e-4, 1.92e-3))):
+        h = G.run_gubser(hip, N)
+        o = G.run_gubser(oracle, N)
+        print(f"N {N}: L2(T) {h[0]:.4e} (oracle {o[0]:.4e}), L2(u) {h[1]:.4e} (oracle {o[1]:.4e}), "
+              f"x<->y {h[2]:.1e}", flush=True)
+        assert abs(h[0] - eT) < 0.005 * 10 ** np.floor(np.log10(eT))
+        assert abs(h[1] - eU) < 0.006 * 10 ** np.floor(np.log10(eU))
+        assert abs(h[0] - o[0]) <= 1e-9 * o[0] and abs(h[1] - o[1]) <= 1e-9 * o[1]
+        assert h[2] < 1e-12
+
+
+def test_bjorken_gate_g0_on_gpu(oracle, hip):
+    ou = OracleUnit(oracle)
+    eos = F.LatticeHRGEOS()
+    r = G.run_bjorken(hip, 0.02)
+    assert r["nprimfail"] == 0
+    assert r["maxDerr"] < 1e-13 and r["maxU"] == 0.0 and r["Espread"] < 1e-13
+    r = G.run_bjorken(hip, 0.005)
+    T, n, e = G._bjorken_exact(ou, eos, 0.5, -4.2 * 0.5, 0.4, r["tau"])
+    assert abs(r["E"] - e) / e < 5e-4 and abs(r["T"] - T) / T < 5e-4 and abs(r["n"] - n) / n < 5e-4
+    o = G.run_bjorken(oracle, 0.005)
+    assert abs(r["T"] - o["T"]) <= 1e-9 * o["T"] and abs(r["E"] - o["E"]) <= 1e-9 * o["E"]
+
+
+@pytest.mark.parametrize("zs", [0.05, 0.15, 0.30])
+def test_bulk_bjorken_gate_g0b_on_gpu(oracle, hip, zs):
+    tauf = 6.0
+    h = D.run_bulk(hip, zs, tauf)
+    o = D.run_bulk(oracle, zs, tauf)
+    ee, Pie, Te = D.bulk_referee(None, None, zs, 15.0, tauf)
+    dT = abs(h["T"] - Te) / Te
+    dPi = abs(h["Pi"] - Pie) / abs(Pie)
+    print(f"zeta/s {zs}: dT {dT:.2e} dPi {dPi:.2e}; vs oracle T {abs(h['T'] - o['T']) / o['T']:.1e} "
+          f"Pi {abs(h['Pi'] - o['Pi']) / abs(o['Pi']):.1e}", flush=True)
+    assert h["Tspread"] < 1e-12
+    assert dT < 1e-3 and dPi < 1e-2 and abs(h["Pi"]) / h["P"] > 0.05
+    assert round(100 * dPi, 2) == D.G0B_PUBLISHED_DPI[zs]
+    assert abs(h["T"] - o["T"]) <= 1e-9 * o["T"] and abs(h["Pi"] - o["Pi"]) <= 1e-9 * abs(o["Pi"])
+
+
+def test_viscous_bjorken_gate_g2c_on_gpu(oracle, hip):
+    h = D.run_visc_bjorken(hip)
+    o = D.run_visc_bjorken(oracle, referee=False)
+    print(f"e rel {h['eerr']:.2e} pi^xx rel {h['perr']:.2e}; vs oracle e {abs(h['e'] - o['e']) / o['e']:.1e}",
+          flush=True)
+    assert h["primfail"] == 0 and h["maxU"] == 0.0 and h["spread"] < 1e-12 and h["shear_res"] < 1e-12
+    assert h["eerr"] < 2e-3 and h["perr"] < 2e-2
+    assert abs(h["e"] - o["e"]) <= 1e-9 * o["e"] and abs(h["pixx"] - o["pixx"]) <= 1e-9 * abs(o["pixx"])
+    ideal = D.run_visc_bjorken(hip, eta_over_s=0.0, referee=False)
+    assert h["pieta"] < 0.0 and h["e"] > ideal["e"]
+
+
+def test_charge_gates_g3_on_gpu(oracle, hip):
+    per = D.run_bump(hip, bc=F.FIVO_BC_PERIODIC)
+    dper = abs(per["Q1"] - per["Q0"]) / abs(per["Q0"])
+    print(f"periodic charge drift {dper:.2e}", flush=True)
+    assert per["r"]["nprimfail"] == 0 and dper < 1e-12
+    on = D.run_bump(hip)
+    ref = D.run_bump(oracle)
+    g, m = on["g"], on["m"]
+    sx, sy = g.interior()
+    inside = np.zeros((g.Nxtot, g.Nytot), bool)
+    inside[sx, sy] = True
+    Ui = on["U"][inside.ravel()]
+    Vi = D.transpose_rows(on["U"], g, m.layout)[inside.ravel()]
+    worst = np.max(np.abs(Ui - Vi) / np.maximum(np.maximum(np.abs(Ui), np.abs(Vi)), 1e-30))
+    assert worst < 1e-12
+    Ro = ref["U"][inside.ravel()]
+    # per-row scale; S^x, S^y on the scale of the energy row (same units): they
+    # are zero up to round-off here (uniform T, a radial charm bump exerts no
+    # pressure), ~1e-17 on one side and exactly 0 on the other
+    scale = np.max(np.abs(Ro), axis=0)
+    for k in ("Sx", "Sy"):
+        scale[m.layout.idx(k)] = max(scale[m.layout.idx(k)], np.max(np.abs(Ro[:, m.layout.idx("E")])))
+    assert np.max(np.abs(Ui - Ro) / scale) <= 1e-9
+
+
+def test_full_sectors_mirror_symmetry_on_gpu(hip):
+    g = F.make_grid2d(32, 32, xmax=8.0, ymax=8.0)
+    m = D.full_model()
+    s = F.Solver(g, m, lib=hip)
+    s.init_from_prims(0.6, **D.symmetric_event(g))
+    r = s.run_sim_2d(tau0=0.6, taufinal=1.2, integrator="ssprk3")
+    assert r["ok"]
+    U = s.get_state()
+    sx, sy = g.interior()
+    inside = np.zeros((g.Nxtot, g.Nytot), bool)
+    inside[sx, sy] = True
+    Ui = U[inside.ravel()]
+    Vi = D.transpose_rows(U, g, m.layout)[inside.ravel()]
+    asym = np.max(np.abs(Ui - Vi), axis=0) / np.maximum(np.max(np.abs(Ui), axis=0), 1e-300)
+    print("x<->y asymmetry", float(np.max(asym)), flush=True)
+    assert np.max(asym) < 1e-9

@@ -36,10 +36,10 @@ def bulk_model(zs):
                             tauPi_coeff=15.0, enable_diff=False)
 
 
-def run_bulk(oracle, zs, tauf, N=24, CFL=0.15):
+def run_bulk(lib, zs, tauf, N=24, CFL=0.15):
     g = F.make_grid2d(N, N, xmax=4.0, ymax=4.0)
     m = bulk_model(zs)
-    s = F.Solver(g, m, lib=oracle)
+    s = F.Solver(g, m, lib=lib)
     s.init_from_prims(TAU0, **ic.uniform(g, T0, ALPHA))
     r = s.run_sim_2d(tau0=TAU0, taufinal=tauf, CFL=CFL, CFLtau=0.05, integrator="ssprk2")
     assert r["ok"]
@@ -54,9 +54,14 @@ def run_bulk(oracle, zs, tauf, N=24, CFL=0.15):
 
 
 @functools.lru_cache(maxsize=None)
-def _referee(zs, Cz, tauf, n):
+def _oracle_unit():
     from conftest import ORACLE_LIB
-    return bulk_referee(OracleUnit(F.Lib(ORACLE_LIB, "orc_")), F.LatticeHRGEOS(), zs, Cz, tauf, n, cached=False)
+    return OracleUnit(F.Lib(ORACLE_LIB, "orc_"))
+
+
+@functools.lru_cache(maxsize=None)
+def _referee(zs, Cz, tauf, n):
+    return bulk_referee(_oracle_unit(), F.LatticeHRGEOS(), zs, Cz, tauf, n, cached=False)
 
 
 def bulk_referee(ou, eos, zs, Cz, tauf, n=4000, cached=True):
@@ -188,11 +193,11 @@ def visc_referee(ou, eos, eta_over_s, Cs, T0v, alpha0, tau0, tauf, nsub=3000):
     return y[0], y[1]
 
 
-def run_visc_bjorken(oracle, CFLtau=0.004, tau0=0.6, tauf=3.0, T0v=0.45, alpha0=-4.2, eta_over_s=0.15,
+def run_visc_bjorken(lib, CFLtau=0.004, tau0=0.6, tauf=3.0, T0v=0.45, alpha0=-4.2, eta_over_s=0.15,
                      referee=True):
     g = F.make_grid2d(20, 20, xmax=5.0, ymax=5.0)
     m = F.build_model_2d(eos=F.LatticeHRGEOS(), enable_shear=True, eta_over_s=eta_over_s, tauShear_coeff=0.2)
-    s = F.Solver(g, m, lib=oracle)
+    s = F.Solver(g, m, lib=lib)
     assert s.init_from_prims(tau0, **ic.uniform(g, T0v, alpha0)) == 0
     r = s.run_sim_2d(tau0=tau0, taufinal=tauf, CFL=0.2, CFLtau=CFLtau)
     assert r["ok"]
@@ -209,8 +214,7 @@ def run_visc_bjorken(oracle, CFLtau=0.004, tau0=0.6, tauf=3.0, T0v=0.45, alpha0=
                spread=(Ein.max() - Ein.min()) / abs(Ein.max()), e=U[i0, L.idx("E")],
                pixx=U[i0, L.idx("pixx")], pieta=U[i0, L.idx("pieta")])
     if referee:
-        ou = OracleUnit(oracle)
-        e_ref, pxx_ref = visc_referee(ou, F.LatticeHRGEOS(), eta_over_s, 0.2, T0v, alpha0, tau0, r["tau"])
+        e_ref, pxx_ref = visc_referee(_oracle_unit(), F.LatticeHRGEOS(), eta_over_s, 0.2, T0v, alpha0, tau0, r["tau"])
         out.update(e_ref=e_ref, pixx_ref=pxx_ref, eerr=abs(out["e"] - e_ref) / e_ref,
                    perr=abs(out["pixx"] - pxx_ref) / max(abs(pxx_ref), 1e-12))
     return out
@@ -270,14 +274,14 @@ def transpose_rows(U, g, layout):
     return V.reshape(-1, nv)
 
 
-def run_bump(oracle, bc=F.FIVO_BC_OUTFLOW, enable_diff=True, N=48, box=8.0, tau0=0.6, tauf=2.0):
+def run_bump(lib, bc=F.FIVO_BC_OUTFLOW, enable_diff=True, N=48, box=8.0, tau0=0.6, tauf=2.0):
     """G3's Gaussian fugacity bump on a uniform T = 0.40 background
     (test/test_charge2d.jl:104-134): alpha = -4.5 + 1.5 exp(-(r/2)^2)."""
     g = F.make_grid2d(N, N, xmax=box, ymax=box)
     m = F.build_model_2d(eos=F.LatticeHRGEOS(), enable_diff=enable_diff, kappa_coeff=0.1163, tauN_coeff=1.0)
     X, Y = np.meshgrid(g.xC, g.yC, indexing="ij")
     alpha = -4.5 + 1.5 * np.exp(-(np.hypot(X, Y) / 2.0) ** 2)
-    s = F.Solver(g, m, lib=oracle)
+    s = F.Solver(g, m, lib=lib)
     assert s.init_from_prims(tau0, T=np.full(g.Ntot, 0.40), alpha=alpha.ravel(), bc=bc) == 0
     sx, sy = g.interior()
     Q0 = s.get_state()[:, 0].reshape(g.Nxtot, g.Nytot)[sx, sy].sum()
