@@ -187,10 +187,13 @@ def ensemble_bench(lib, name, steps, warmup, net):
             torch.cuda.synchronize()
             wall = time.perf_counter() - t0
         assert all(r["ok"] and r["nsteps"] == steps for r in res), path
-        T = [np.exp(s.get_work(("yT",))["yT"]) for s in x.events]
+        W = [s.get_work(("yT", "ok")) for s in x.events]
+        T = [np.exp(w["yT"]) for w in W]
+        OK = [w["ok"] for w in W]
         if Tref is None:
-            Tref = T
+            Tref, OKref = T, OK
         rel = [np.abs(a - b) / b for a, b in zip(T, Tref)]
+        flips = int(sum(int(np.sum(a != b)) for a, b in zip(OK, OKref)))
         dT = max(float(np.max(r)) for r in rel)
         ncells_gt = int(sum(int(np.sum(r > 1e-9)) for r in rel))
         nev_gt = int(sum(int(np.any(r > 1e-9)) for r in rel))
@@ -199,7 +202,7 @@ def ensemble_bench(lib, name, steps, warmup, net):
         info.update({"cell_updates_per_s": cells * steps / wall, "event_steps_per_s": e["nev"] * steps / wall,
                      "ms_per_step_all_events": wall / steps * 1e3, "max_dT_over_T_vs_plain": dT,
                      "cells_dT_over_1e-9": ncells_gt, "events_with_cells_dT_over_1e-9": nev_gt,
-                     "max_dT_over_T_other_events": dT_clean,
+                     "max_dT_over_T_other_events": dT_clean, "cells_recovery_ok_flipped": flips,
                      "newton_iters_per_cell_update": sum(r["newton_iters"] for r in res) / (cells * steps)})
         if path == "batched_learned":
             info["learned_starts"] = nl
@@ -332,6 +335,19 @@ def cpu_baseline(cfg, budget_s=12.0):
                       f"{wall:.1f} s wall, OpenMP over x-columns"}
 
 
+def kernel_times(lib, ctx):
+    """{kernel: (total ms, launches)} from the library's HIP-event timers (one
+    event pair around each launch on the library stream)."""
+    tms = {}
+    for cat, name in ((0, "k_prim"), (5, "k_prim_coop"), (6, "k_prim_seed"), (1, "k_faces"), (2, "k_bc"),
+                      (3, "k_relax")):
+        tot = C.c_double(0)
+        cnt = lib.kernel_time(ctx, cat, C.byref(tot))
+        if cnt > 0:
+            tms[name] = (tot.value, cnt)
+    return tms
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -367,13 +383,7 @@ def main():
     log(f"{cfg['workload']} plain: {value:.3e} cell-updates/s")
     # dominant kernel from the event timings of the timed region (HIP events on
     # the library's stream, one pair around each launch)
-    tms = {}
-    for cat, name in ((0, "k_prim"), (5, "k_prim_coop"), (6, "k_prim_seed"), (1, "k_faces"), (2, "k_bc"),
-                      (3, "k_relax")):
-        tot = C.c_double(0)
-        cnt = lib.kernel_time(s.ctx, cat, C.byref(tot))
-        if cnt > 0:
-            tms[name] = (tot.value, cnt)
+    tms = kernel_times(lib, s.ctx)
     dom = max(tms, key=lambda k: tms[k][0])
     avg_ms = tms[dom][0] / tms[dom][1]
     alg = kernel_alg_bytes(dom, cfg, g, r, tms[dom][1])
@@ -410,11 +420,13 @@ def main():
             g2, m2, s2 = make_problem(cfg, lib)
             s2.set_learned_seed(net)
             r = run_steps(s2, cfg["tau0"], max(a.warmup, 1), cfg, True)
+            lib.enable_timing(s2.ctx, 1)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             r = run_steps(s2, r["tau"], a.steps, cfg, False)
             torch.cuda.synchronize()
             wl = time.perf_counter() - t0
+            tml = kernel_times(lib, s2.ctx)
             Tp = np.exp(s.get_work(("yT",))["yT"])
             Tl = np.exp(s2.get_work(("yT",))["yT"])
             out["plain_learned_512"] = {
@@ -422,7 +434,8 @@ def main():
                 "speedup_vs_plain": wall / wl, "max_dT_over_T_vs_plain": float(np.max(np.abs(Tl - Tp) / Tp)),
                 "learned_starts": r["nlearned"], "fallback_rate": r["nlearned_fallback"] / max(r["nlearned"], 1),
                 "newton_iters_per_cell_update": r["newton_iters"] / (ncell * a.steps),
-                "newton_iters_per_cell_update_plain": r_plain["newton_iters"] / (ncell * a.steps)}
+                "newton_iters_per_cell_update_plain": r_plain["newton_iters"] / (ncell * a.steps),
+                "kernel_ms_per_step": {k: round(v[0] / a.steps, 5) for k, v in tml.items()}}
             s2.close()
             log(f"{cfg['workload']} plain+learned: {out['plain_learned_512']['cell_updates_per_s']:.3e}")
             out["batched_learned_512"] = batched_learned_single(cfg, lib, net, a.steps, max(a.warmup, 1), Tp, wall)

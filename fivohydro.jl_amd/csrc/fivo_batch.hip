@@ -42,8 +42,10 @@ int build_maps(Batch& b) {
         v.nb[KB_INTERIOR] = nblk((int64_t)p.Nx * p.Ny, 256);
         v.nb[KB_GHOST] = nblk(2 * (int64_t)p.ng * p.Nytot + 2 * (int64_t)p.ng * p.Nx, 256);
         v.nb[KB_FACES] = batch_face_blocks(p, b.tile, &v.tiles_y, &v.nmain);
-        // 8 cells in flight per 64-lane block; enough groups for a warm pass's queue
-        v.nb[KB_COOP] = std::max(4, std::min(128, p.Ntot / 256));
+        // 8 cells in flight per 64-lane block; one group per queued cell of a warm
+        // pass (the queue holds ~0.5 % of the cells), so no group solves two cells
+        // one after the other; blocks past the queue exit at once
+        v.nb[KB_COOP] = std::max(4, std::min(1024, p.Ntot / 64));
         for (int k = 0; k < KB_N; ++k) {
             v.b0[k] = b.nblocks[k];
             b.nblocks[k] += v.nb[k];

@@ -6,6 +6,10 @@
                       drawn from a 2-D Woods-Saxon profile, Gamma(k=1) amplitudes,
                       charm density from the binary-collision density of the same
                       spots (the pattern of examples2d/04_fluctuating_event.jl:66-73)
+  grid_csv            a genuinely 2-D IC from an `x,y,T0,alpha0` grid file,
+                      initialize_from_grid_csv! (main2D.jl:205-301): optional
+                      Gaussian blur of the file grid, bilinear interpolation,
+                      vacuum outside the file, radial taper
 
 Every function returns arrays over all Ntot cells (ghosts included), flattened in
 the library's flat order i = ix*Nytot + iy, ready for Solver.init_from_prims.
@@ -127,3 +131,102 @@ def hotspot_event(g, seed, nspots=150, sigma=0.5, Tpeak=0.45, R=6.6, a=0.55, eos
     alpha = np.maximum(alpha, alpha_floor)
     n = g.Ntot
     return dict(T=T.ravel(), alpha=alpha.ravel(), ux=np.zeros(n), uy=np.zeros(n))
+
+
+# ---- 2-D grid file -------------------------------------------------------------------
+def _smoothstep01_5(x):
+    """src/utils.jl:27-29: 0 below 0, 1 above 1, x^3 (10 - 15x + 6x^2) between."""
+    x = np.asarray(x, dtype=np.float64)
+    mid = (x * x * x) * ((10.0 - 15.0 * x) + 6.0 * (x * x))
+    return np.where(x <= 0, 0.0, np.where(x >= 1, 1.0, mid))
+
+
+def read_grid_csv(csv):
+    """The file's uniform grid: (xs, ys, T[nx, ny], alpha[nx, ny]) (main2D.jl:241-252).
+    Rows are placed by rounding (x - x_1)/dx to the nearest integer, ties to even."""
+    raw = np.loadtxt(csv, delimiter=",", skiprows=1, ndmin=2)
+    xs = np.unique(raw[:, 0])
+    ys = np.unique(raw[:, 1])
+    nx, ny = len(xs), len(ys)
+    if raw.shape[0] != nx * ny:
+        raise ValueError("grid CSV is not a complete uniform grid")
+    dx, dy = xs[1] - xs[0], ys[1] - ys[0]
+    ix = np.rint((raw[:, 0] - xs[0]) / dx).astype(np.int64)
+    iy = np.rint((raw[:, 1] - ys[0]) / dy).astype(np.int64)
+    T = np.zeros((nx, ny))
+    A = np.zeros((nx, ny))
+    T[ix, iy] = raw[:, 2]
+    A[ix, iy] = raw[:, 3]
+    return xs, ys, T, A
+
+
+def _blur_axis(A, sig, axis):
+    """One pass of the normalised Gaussian blur, edge-clamped: samples outside the
+    grid are dropped and the weights renormalised (main2D.jl:259-280), summed in
+    kernel order."""
+    rad = max(1, int(math.ceil(3 * sig)))
+    w = [math.exp(-0.5 * ((k / sig) * (k / sig))) for k in range(-rad, rad + 1)]
+    n = A.shape[axis]
+    acc = np.zeros_like(A)
+    wsum = np.zeros_like(A)
+    idx = np.arange(n)
+    for kk, k in enumerate(range(-rad, rad + 1)):
+        src = idx + k
+        ok = (src >= 0) & (src < n)
+        sh = np.zeros_like(A)
+        if axis == 0:
+            sh[ok, :] = A[src[ok], :]
+            m = ok[:, None] & np.ones_like(A, dtype=bool)
+        else:
+            sh[:, ok] = A[:, src[ok]]
+            m = ok[None, :] & np.ones_like(A, dtype=bool)
+        acc = np.where(m, acc + w[kk] * sh, acc)
+        wsum = np.where(m, wsum + w[kk], wsum)
+    return acc / wsum
+
+
+def grid_csv(g, csv, taper_width=1.0, smooth_fm=0.0):
+    """Primitive fields of initialize_from_grid_csv! (main2D.jl:238-301) on grid `g`:
+    bilinear interpolation of the file's T and alpha (0 on and beyond the file's
+    edges), a radial taper over `taper_width` fm inside the file's smaller half
+    extent (T -> max(w T, T_MIN), alpha -> w alpha), T >= T_MIN; u = 0."""
+    xs, ys, Tg, Ag = read_grid_csv(csv)
+    dx, dy = xs[1] - xs[0], ys[1] - ys[0]
+    if smooth_fm > 0.0:
+        Tg = _blur_axis(_blur_axis(Tg, smooth_fm / dx, 0), smooth_fm / dy, 1)
+        Ag = _blur_axis(_blur_axis(Ag, smooth_fm / dx, 0), smooth_fm / dy, 1)
+    nx, ny = len(xs), len(ys)
+    xlo, xhi, ylo, yhi = xs[0], xs[-1], ys[0], ys[-1]
+    rmax_data = min(xhi, yhi)
+    tw = max(taper_width, 0.0)
+    rt0 = max(rmax_data - tw, 0.0)
+    X, Y = _mesh(g)
+
+    def bilin(A):
+        inside = (X > xlo) & (X < xhi) & (Y > ylo) & (Y < yhi)
+        i = np.clip(np.floor((X - xlo) / dx).astype(np.int64), 0, nx - 2)
+        j = np.clip(np.floor((Y - ylo) / dy).astype(np.int64), 0, ny - 2)
+        tx = (X - xs[i]) / dx
+        ty = (Y - ys[j]) / dy
+        v = ((1 - tx) * (1 - ty) * A[i, j] + tx * (1 - ty) * A[i + 1, j]
+             + (1 - tx) * ty * A[i, j + 1] + tx * ty * A[i + 1, j + 1])
+        return np.where(inside, v, 0.0)
+
+    T = bilin(Tg)
+    a = bilin(Ag)
+    r = np.hypot(X, Y)
+    if tw > 0:
+        tap = r > rt0
+        w = 1.0 - _smoothstep01_5((r - rt0) / tw)
+        T = np.where(tap, np.maximum(w * T, T_MIN), T)
+        a = np.where(tap, w * a, a)
+    T = np.maximum(T, T_MIN)
+    n = g.Ntot
+    return dict(T=T.ravel(), alpha=a.ravel(), ux=np.zeros(n), uy=np.zeros(n))
+
+
+def write_grid_csv(path, xs, ys, T, alpha):
+    """Write an `x,y,T0,alpha0` grid file (the format grid_csv reads)."""
+    X, Y = np.meshgrid(xs, ys, indexing="ij")
+    rows = np.column_stack([X.ravel(), Y.ravel(), np.asarray(T).ravel(), np.asarray(alpha).ravel()])
+    np.savetxt(path, rows, delimiter=",", header="x,y,T0,alpha0", comments="", fmt="%.17g")

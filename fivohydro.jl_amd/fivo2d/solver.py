@@ -151,6 +151,28 @@ class Solver:
                                          float(alpha_floor), int(bc), C.byref(nbad)), "init_hotspots")
         return nbad.value
 
+    def initialize_from_grid_csv(self, tau0, csv, taper_width=1.0, smooth_fm=0.0, bc=_abi.FIVO_BC_OUTFLOW):
+        """initialize_from_grid_csv! (main2D.jl:238-301): the file's T and alpha
+        interpolated onto the grid (fivo2d.ic.grid_csv), set_cell! on every cell and
+        finalize_ic! on the device. Returns {"nbad", "nvacuum"}: cells whose
+        set_cell! failed, split like the reference into vacuum (T <= 10 T_MIN) and
+        cells that had matter."""
+        from .ic import grid_csv
+        prims = grid_csv(self.g, csv, taper_width=taper_width, smooth_fm=smooth_fm)
+        vac = prims["T"] <= 10 * T_MIN
+        if vac.any():
+            # failures among the cells with matter: the same build with the vacuum
+            # cells at a state that always succeeds; then the real build
+            # (a failed set_cell! leaves its cell as it was, so the state is put back
+            # between the two)
+            probe = dict(prims, T=np.where(vac, 1.0, prims["T"]), alpha=np.where(vac, 0.0, prims["alpha"]))
+            U0 = self.get_state()
+            nbad = self.init_from_prims(tau0, bc=bc, **probe)
+            self.set_state(U0)
+            nfail_all = self.init_from_prims(tau0, bc=bc, **prims)
+            return {"nbad": nbad, "nvacuum": nfail_all - nbad}
+        return {"nbad": self.init_from_prims(tau0, bc=bc, **prims), "nvacuum": 0}
+
     def set_learned_seed(self, mlp):
         """Start cells without a warm start from `mlp`'s guess (None: the cold seed)."""
         self._chk(self.lib.set_learned_seed(self.ctx, None if mlp is None else mlp.m), "set_learned_seed")
