@@ -103,7 +103,10 @@ int record_step(Batch& b) {
         BCHK(&b, launch_batch_relax(b));
         BCHK(&b, launch_batch_ctl(b, 20));
     }
-    if (b.p0.enable_shear) BCHK(&b, launch_batch_project_shear(b));
+    if (b.p0.enable_shear) {
+        BCHK(&b, launch_batch_project_shear(b));
+        BCHK(&b, launch_batch_ctl(b, -2));
+    }
     BCHK(&b, launch_batch_ctl(b, -1));
     BCHK(&b, launch_batch_ctl(b, 21));
     return FIVO_OK;

@@ -85,8 +85,10 @@ __global__ __launch_bounds__(256) void kb_prim(BatchRun R, int phase) {
 }
 
 // ---- primitive pass: the event's queued cells, GW lanes per cell ------------------------
+// Two wavefronts per SIMD (registers capped at 256, a few spilled): an ensemble's
+// queue holds tens of thousands of cells, so occupancy beats per-wave latency here.
 template <int EK, bool NCH>
-__global__ __launch_bounds__(64) void kb_coop(BatchRun R, int phase) {
+__global__ __launch_bounds__(64, 2) void kb_coop(BatchRun R, int phase) {
     const int e = block_event(R, KB_COOP);
     CCtl c = (CCtl)(R.ctl + e);
     if (!ev_active(c, phase)) return;
@@ -292,9 +294,7 @@ __global__ __launch_bounds__(64) void kb_ctl(BatchRun R, int what) {
     }
     if (what == 21) {   // step end: shear residual of the accepted step
         if (c.accepted && E.p.enable_shear) {
-            const int nbi = E.nb[KB_INTERIOR];
-            double m = 0.0;
-            for (int b = 0; b < nbi; ++b) m = E.partials[2 * b] > m ? E.partials[2 * b] : m;
+            const double m = c.shear_step;
             c.max_shear = m > c.max_shear ? m : c.max_shear;
         }
         if (c.state == EV_RUNNING && !(c.tau < c.taufinal && c.nsteps < c.max_steps) && c.fresh) c.state = EV_DONE;
@@ -303,6 +303,22 @@ __global__ __launch_bounds__(64) void kb_ctl(BatchRun R, int what) {
     }
 }
 __global__ void kb_zero_running(BatchRun R) { R.bs->nrunning = 0; }
+
+// the maximum of an event's per-block shear residuals after the projection, one
+// wavefront per event (the maximum of a set: the same value in any order; NaN
+// partials are skipped, like the sequential `a > m ? a : m`)
+__global__ __launch_bounds__(64) void kb_shear_max(BatchRun R) {
+    const int e = blockIdx.x;
+    EvCtl& c = R.ctl[e];
+    const EvView& E = R.ev[e];
+    double m = 0.0;
+    if (c.accepted) {
+        const int nbi = E.nb[KB_INTERIOR];
+        for (int b = threadIdx.x; b < nbi; b += 64) m = E.partials[2 * b] > m ? E.partials[2 * b] : m;
+    }
+    m = wave_max_pos(m);
+    if (threadIdx.x == 0) c.shear_step = m;
+}
 
 // ---- launchers -------------------------------------------------------------------------------
 #define EOS_DISPATCH_B(b, BODY)                                                      \
@@ -381,8 +397,10 @@ hipError_t launch_batch_project_shear(Batch& b) {
                        b.shear_project);
     return hipGetLastError();
 }
+// what >= 0: kb_ctl's phases; -1 clears the running count; -2 the shear residual maxima
 hipError_t launch_batch_ctl(Batch& b, int what) {
     if (what == -1) hipLaunchKernelGGL(kb_zero_running, dim3(1), dim3(1), 0, b.stream, b.run);
+    else if (what == -2) hipLaunchKernelGGL(kb_shear_max, dim3(b.nev), dim3(64), 0, b.stream, b.run);
     else hipLaunchKernelGGL(kb_ctl, dim3((b.nev + 63) / 64), dim3(64), 0, b.stream, b.run, what);
     return hipGetLastError();
 }

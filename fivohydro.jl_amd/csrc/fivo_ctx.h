@@ -173,6 +173,7 @@ struct EvView {
 enum { EV_RUNNING = 0, EV_DONE = 1, EV_FAILED = 2 };
 struct EvCtl {
     double tau, D, taufinal, amax, max_shear;
+    double shear_step;  // shear residual maximum of the current replay's projection
     int cur;        // index of the accepted state among U[0..3]
     int state;      // EV_*
     int fresh;      // next attempt starts a new step (history snapshot, new dt)

@@ -550,8 +550,11 @@ __device__ __forceinline__ void relax_cell(const double* __restrict__ Q, double*
     }
 }
 
+// Two wavefronts per SIMD: the cell body needs ~290 registers, so at one wave per
+// SIMD nothing hides its latency; capping at 256 spills a few values to scratch
+// (measured 4-5 % faster per step on ensembles, ~1 % on one 512^2 event).
 template <int EK>
-__global__ __launch_bounds__(256) void k_relax(const double* __restrict__ Q, double* __restrict__ U, Params p,
+__global__ __launch_bounds__(256, 2) void k_relax(const double* __restrict__ Q, double* __restrict__ U, Params p,
                                                DevWork w, double tau, double D) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < p.Ntot) relax_cell<EK>(Q, U, p, w, tau, D, i);
@@ -561,7 +564,7 @@ __global__ __launch_bounds__(256) void k_relax(const double* __restrict__ Q, dou
 typedef const __attribute__((address_space(4))) EvView* CEvR;
 typedef const __attribute__((address_space(4))) EvCtl* CCtlR;
 template <int EK>
-__global__ __launch_bounds__(256) void kb_relax(BatchRun R) {
+__global__ __launch_bounds__(256, 2) void kb_relax(BatchRun R) {
     const int e = __builtin_amdgcn_readfirstlane(((const __attribute__((address_space(4))) int*)R.map[KB_CELLS])[blockIdx.x]);
     CCtlR c = (CCtlR)(R.ctl + e);
     if (!c->accepted) return;
