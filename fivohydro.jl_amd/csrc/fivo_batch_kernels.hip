@@ -140,7 +140,7 @@ __global__ __launch_bounds__(64) void kb_seed(BatchRun R, int phase) {
 //   1: U1 = U + D k(U, tau)            2: U2 = 3/4 U + 1/4 (U1 + D k(U1, tau + D))
 //   3: U  = 1/3 U + 2/3 (U2 + D k(U2, tau + D/2)), each sanitised at tau + D
 template <int EK, bool NCH, int TX, int TY>
-__global__ __launch_bounds__(TX* TY) void kb_faces(BatchRun R, int phase) {
+__global__ __launch_bounds__((face_threads<TX, TY>())) void kb_faces(BatchRun R, int phase) {
     extern __shared__ double sF[];
     const int e = block_event(R, KB_FACES);
     CCtl c = (CCtl)(R.ctl + e);
@@ -329,16 +329,16 @@ __global__ __launch_bounds__(64) void kb_shear_max(BatchRun R) {
     } while (0)
 
 void batch_tile_shape(int tile, int* tx, int* ty) {
-    static const int T[BATCH_NTILES][2] = {{4, 16}, {8, 16}, {8, 32}, {16, 16}};
+    static const int T[BATCH_NTILES][2] = {{7, 8}, {8, 28}, {12, 20}, {15, 16}};
     *tx = T[tile][0]; *ty = T[tile][1];
 }
 int batch_face_blocks(const Params& p, int tile, int* tiles_y, int* nmain) {
     int nb = 0;
     switch (tile) {
-        case 0: face_grid<4, 16>(p, tiles_y, nmain, &nb); break;
-        case 1: face_grid<8, 16>(p, tiles_y, nmain, &nb); break;
-        case 2: face_grid<8, 32>(p, tiles_y, nmain, &nb); break;
-        default: face_grid<16, 16>(p, tiles_y, nmain, &nb); break;
+        case 0: face_grid<7, 8>(p, tiles_y, nmain, &nb); break;
+        case 1: face_grid<8, 28>(p, tiles_y, nmain, &nb); break;
+        case 2: face_grid<12, 20>(p, tiles_y, nmain, &nb); break;
+        default: face_grid<15, 16>(p, tiles_y, nmain, &nb); break;
     }
     return nb;
 }
@@ -369,17 +369,17 @@ static hipError_t batch_faces_tx(Batch& b, int phase) {
         if (e != hipSuccess) return e;
         lds_set = lds;
     }
-    hipLaunchKernelGGL((kb_faces<EK, NCH, TX, TY>), dim3(b.nblocks[KB_FACES]), dim3(TX * TY), lds, b.stream, b.run,
+    hipLaunchKernelGGL((kb_faces<EK, NCH, TX, TY>), dim3(b.nblocks[KB_FACES]), dim3(face_threads<TX, TY>()), lds, b.stream, b.run,
                        phase);
     return hipGetLastError();
 }
 hipError_t launch_batch_faces(Batch& b, int phase, int tile) {
     hipError_t e = hipSuccess;
     switch (tile) {
-        case 0: EOS_DISPATCH_B(b, e = (batch_faces_tx<EK, NCH, 4, 16>(b, phase))); break;
-        case 1: EOS_DISPATCH_B(b, e = (batch_faces_tx<EK, NCH, 8, 16>(b, phase))); break;
-        case 2: EOS_DISPATCH_B(b, e = (batch_faces_tx<EK, NCH, 8, 32>(b, phase))); break;
-        default: EOS_DISPATCH_B(b, e = (batch_faces_tx<EK, NCH, 16, 16>(b, phase))); break;
+        case 0: EOS_DISPATCH_B(b, e = (batch_faces_tx<EK, NCH, 7, 8>(b, phase))); break;
+        case 1: EOS_DISPATCH_B(b, e = (batch_faces_tx<EK, NCH, 8, 28>(b, phase))); break;
+        case 2: EOS_DISPATCH_B(b, e = (batch_faces_tx<EK, NCH, 12, 20>(b, phase))); break;
+        default: EOS_DISPATCH_B(b, e = (batch_faces_tx<EK, NCH, 15, 16>(b, phase))); break;
     }
     return e;
 }

@@ -398,6 +398,13 @@ struct FaceK {
     int remap;            // XCD-aware tile order (one grid per event)
 };
 // main tiles + extra blocks for the ghost-region faces of a TX x TY tiling
+// Threads of a face-tile block: the tile's cells rounded up to whole wavefronts.
+// A TX x TY tile has 2 TX TY + TX + TY faces; tiles with at most two faces per
+// thread (e.g. 15 x 16, 12 x 20, 8 x 28, 7 x 8) finish the face phase in two
+// rounds, where a 256-cell tile needs a third round for its TX + TY halo faces.
+template <int TX, int TY>
+constexpr int face_threads() { return (TX * TY + 63) / 64 * 64; }
+
 template <int TX, int TY>
 __host__ __device__ inline void face_grid(const Params& p, int* tiles_y, int* nmain, int* nblocks) {
     const int tiles_x = (p.Nx + TX - 1) / TX;
@@ -405,7 +412,8 @@ __host__ __device__ inline void face_grid(const Params& p, int* tiles_y, int* nm
     *nmain = tiles_x * *tiles_y;
     const int ng = p.ng;
     const int nextra = 2 * ng * (p.Nxtot - 1) + 2 * p.Ny * (ng - 1) + 2 * ng * (p.Nytot - 1) + 2 * p.Nx * (ng - 1);
-    *nblocks = *nmain + (nextra + TX * TY - 1) / (TX * TY);
+    constexpr int NT = face_threads<TX, TY>();
+    *nblocks = *nmain + (nextra + NT - 1) / NT;
 }
 
 // One face: left cell i, right cell i + st (st = Nytot for x, 1 for y);
@@ -608,7 +616,7 @@ __device__ __forceinline__ void store_cell(double* __restrict__ U, const Params&
 // MODE 0: write the RHS to f.dUout (rhs_2d!); MODE 1: the RK stage update
 template <int EK, bool NCH, int TX, int TY, int MODE>
 __device__ __forceinline__ void faces_body(const FaceK& K, const int bid, double* __restrict__ sF) {
-    constexpr int NT = TX * TY;
+    constexpr int NT = face_threads<TX, TY>();
     constexpr int NFX = (TX + 1) * TY, NFY = TX * (TY + 1), NF = NFX + NFY;
     const Params& p = K.p;
     const int tid = threadIdx.x;

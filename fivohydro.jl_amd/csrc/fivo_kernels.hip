@@ -51,7 +51,7 @@ __global__ __launch_bounds__(64) void k_prim_seed(const double* __restrict__ U, 
 }
 
 template <int EK, bool NCH, int TX, int TY, int MODE>
-__global__ __launch_bounds__(TX* TY) void k_faces(FaceK K) {
+__global__ __launch_bounds__((face_threads<TX, TY>())) void k_faces(FaceK K) {
     extern __shared__ double sF[];
     faces_body<EK, NCH, TX, TY, MODE>(K, blockIdx.x, sF);
 }
@@ -284,15 +284,17 @@ static hipError_t faces_tx(Ctx& c, const FaceLaunch& f) {
         if (e != hipSuccess) return e;
         lds_set = lds;
     }
-    if (f.mode == 0) hipLaunchKernelGGL((k_faces<EK, NCH, TX, TY, 0>), dim3(nb), dim3(TX * TY), lds, c.stream, K);
-    else hipLaunchKernelGGL((k_faces<EK, NCH, TX, TY, 1>), dim3(nb), dim3(TX * TY), lds, c.stream, K);
+    if (f.mode == 0) hipLaunchKernelGGL((k_faces<EK, NCH, TX, TY, 0>), dim3(nb), dim3(face_threads<TX, TY>()), lds, c.stream, K);
+    else hipLaunchKernelGGL((k_faces<EK, NCH, TX, TY, 1>), dim3(nb), dim3(face_threads<TX, TY>()), lds, c.stream, K);
     return hipGetLastError();
 }
 
 hipError_t launch_faces(Ctx& c, const FaceLaunch& f) {
     hipError_t e = hipSuccess;
-    if (c.tile == 0) EOS_DISPATCH(c, e = (faces_tx<EK, NCH, 8, 32>(c, f)));
-    else EOS_DISPATCH(c, e = (faces_tx<EK, NCH, 4, 16>(c, f)));
+    // 15 x 16 cells (one 256-lane block, two face rounds) from 300^2 up; 7 x 8 (one
+    // wavefront) below, where larger tiles would leave CUs without blocks
+    if (c.tile == 0) EOS_DISPATCH(c, e = (faces_tx<EK, NCH, 15, 16>(c, f)));
+    else EOS_DISPATCH(c, e = (faces_tx<EK, NCH, 7, 8>(c, f)));
     return e;
 }
 

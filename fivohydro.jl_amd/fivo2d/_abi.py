@@ -20,7 +20,7 @@ FIVO_BC_PERIODIC = 1
 FIVO_SSPRK2 = 2
 FIVO_SSPRK3 = 3
 FIVO_BATCH_NTILES = 4
-BATCH_TILES = ((4, 16), (8, 16), (8, 32), (16, 16))   # candidate face tiles, FIVO_BATCH_NTILES of them
+BATCH_TILES = ((7, 8), (8, 28), (12, 20), (15, 16))   # candidate face tiles, FIVO_BATCH_NTILES of them
 
 # Terms field order of src/terms.jl:82-106 -> bit index
 TERM_NAMES = (

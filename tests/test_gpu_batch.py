@@ -61,7 +61,7 @@ def run_ensemble(hip, m, evs, tile=None, autotune=False):
     return out
 
 
-@pytest.mark.parametrize("tile", [(4, 16), (8, 32)])
+@pytest.mark.parametrize("tile", [(7, 8), (15, 16)])
 def test_ensemble_equals_one_event_runs(hip, tile):
     m = full_model()
     evs = events()
