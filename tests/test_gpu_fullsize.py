@@ -1,0 +1,128 @@
+"""GPU checks at the workloads' full sizes.
+
+- per-RHS parity (<= 1e-12 per field) at 512^2 full sectors (the benchmark's
+  event) on the initial state and on a state the GPU evolved for 3 steps
+  (dissipative fields and warm starts copied into the oracle);
+- the 256^2 viscous + charm event (SSPRK2 with MOOD): 3 steps against the oracle,
+  same step/halving/MOOD counts, final T and u within 1e-9;
+- size-independent properties at 512^2 with every sector on: on a periodic box
+  the interior sum of tau J^tau is conserved to round-off; an event symmetric
+  under x <-> y stays symmetric (every state row < 1e-9, the all-sector gate's
+  bound, test/test_production_allsectors2d.jl:123-125);
+- the 64-event ensemble (grids 128^2 ... 384^2): batched runs are bit-identical
+  to the events run one by one.
+"""
+import numpy as np
+import pytest
+
+import fivo2d as F
+from fivo2d import ic
+from helpers import rel_diff
+import test_oracle_dissipative_gates as D
+
+pytestmark = pytest.mark.gpu
+
+
+def full_model(m2=True):
+    return F.build_model_2d(eos=F.LatticeHRGEOS(), enable_shear=True, eta_over_s=0.08, tauShear_coeff=0.2,
+                            enable_bulk=True, zeta_over_s=0.1, tauPi_coeff=15.0, enable_diff=True,
+                            kappa_coeff=0.1163, pi_clip_factor=1.0, consistent_m2=m2)
+
+
+def c3_event():
+    g = F.make_grid2d(512, 512, xmax=15.0, ymax=15.0)
+    return g, ic.hotspot_event(g, seed=12345, nspots=300, sigma=0.4, Tpeak=0.45)
+
+
+def _rhs_compare(so, sh, tau):
+    dUo, co = so.rhs_2d(tau)
+    dUh, ch = sh.rhs_2d(tau)
+    r = rel_diff(dUh, dUo)
+    print("per-field dU rel diff", np.max(r), flush=True)
+    assert np.all(np.isfinite(dUh)) and np.max(r) <= 1e-12, r
+    assert co["nprimfail"] == ch["nprimfail"] and co["nvacuum"] == ch["nvacuum"]
+    assert abs(co["amax"] - ch["amax"]) <= 1e-13 * co["amax"]
+    wo, wh = so.get_work(("ok",)), sh.get_work(("ok",))
+    assert np.array_equal(wo["ok"], wh["ok"])
+
+
+def test_rhs_parity_512_full_sectors(oracle, hip):
+    g, prims = c3_event()
+    m = full_model()
+    sh = F.Solver(g, m, lib=hip)
+    sh.init_from_prims(0.4, **prims)
+    so = F.Solver(g, m, lib=oracle)
+    so.set_state(sh.get_state())
+    _rhs_compare(so, sh, 0.4)
+    # an evolved state: three GPU steps, then the same state, warm starts and
+    # histories in both
+    r = sh.run_sim_2d(tau0=0.4, taufinal=1e9, CFL=0.2, CFLtau=0.05, integrator="ssprk3", max_steps=3)
+    assert r["ok"] and r["nsteps"] == 3
+    so.set_state(sh.get_state())
+    so.set_work(sh.get_work())
+    _rhs_compare(so, sh, r["tau"])
+
+
+def test_c2_event_three_steps_vs_oracle(oracle, hip):
+    g = F.make_grid2d(256, 256, xmax=12.0, ymax=12.0)
+    prims = ic.hotspot_event(g, seed=12345, nspots=150, sigma=0.5, Tpeak=0.45)
+    m = full_model(m2=False)
+    out = []
+    for lib in (oracle, hip):
+        s = F.Solver(g, m, lib=lib)
+        s.init_from_prims(0.6, **prims)
+        r = s.run_sim_2d(tau0=0.6, taufinal=1e9, CFL=0.2, CFLtau=0.05, integrator="ssprk2", max_steps=3)
+        assert r["ok"]
+        out.append((r, s.fields_2d()))
+    (ro, fo), (rh, fh) = out
+    assert (ro["nsteps"], ro["nhalvings"], ro["nmood"], ro["nprimfail"]) == \
+        (rh["nsteps"], rh["nhalvings"], rh["nmood"], rh["nprimfail"])
+    for k in ("T", "ux", "uy"):
+        d = np.max(np.abs(fh[k] - fo[k])) / np.max(np.abs(fo[k]))
+        print(f"{k}: {d:.2e}", flush=True)
+        assert d <= 1e-9, (k, d)
+
+
+def test_512_periodic_charge_and_mirror_symmetry(hip):
+    g = F.make_grid2d(512, 512, xmax=15.0, ymax=15.0)
+    m = full_model()
+    s = F.Solver(g, m, lib=hip)
+    s.init_from_prims(0.4, **D.symmetric_event(g, seed=12345, nspots=300, sigma=0.4), bc=F.FIVO_BC_PERIODIC)
+    sx, sy = g.interior()
+    Q0 = s.get_state()[:, 0].reshape(g.Nxtot, g.Nytot)[sx, sy].sum()
+    r = s.run_sim_2d(tau0=0.4, taufinal=1e9, CFL=0.2, CFLtau=0.05, integrator="ssprk3", max_steps=5,
+                     bc=F.FIVO_BC_PERIODIC)
+    assert r["ok"] and r["nsteps"] == 5
+    U = s.get_state()
+    Q1 = U[:, 0].reshape(g.Nxtot, g.Nytot)[sx, sy].sum()
+    print("charge drift", abs(Q1 - Q0) / Q0, flush=True)
+    assert abs(Q1 - Q0) / Q0 < 1e-12
+    inside = np.zeros((g.Nxtot, g.Nytot), bool)
+    inside[sx, sy] = True
+    Ui = U[inside.ravel()]
+    Vi = D.transpose_rows(U, g, m.layout)[inside.ravel()]
+    asym = np.max(np.abs(Ui - Vi), axis=0) / np.maximum(np.max(np.abs(Ui), axis=0), 1e-300)
+    print("x<->y asymmetry", float(np.max(asym)), flush=True)
+    assert np.max(asym) < 1e-9
+
+
+def test_64_event_ensemble_bit_identical_to_one_by_one(hip):
+    rng = np.random.Generator(np.random.Philox(2024))
+    sizes = rng.choice((128, 192, 256, 320, 384), size=64)
+    nsp = rng.integers(50, 301, size=64)
+    m = full_model()
+    grids = [F.make_grid2d(int(n), int(n), xmax=15.0, ymax=15.0) for n in sizes]
+    leg = dict(taufinal=1e9, CFL=0.2, CFLtau=0.05, max_steps=2)
+    x = F.Ensemble(grids, m, lib=hip)
+    for k, s in enumerate(x.events):
+        s.init_hotspots(0.4, seed=2024000 + k, nspots=int(nsp[k]), sigma=0.4, Tpeak=0.45)
+    res = x.run_sim_2d(tau0=0.4, **leg)
+    Ub = [s.get_state() for s in x.events]
+    x.close()
+    y = F.Ensemble(grids, m, lib=hip)
+    for k, s in enumerate(y.events):
+        s.init_hotspots(0.4, seed=2024000 + k, nspots=int(nsp[k]), sigma=0.4, Tpeak=0.45)
+        r = s.run_sim_2d(tau0=0.4, integrator="ssprk3", **leg)
+        assert r["nsteps"] == res[k]["nsteps"] and r["tau"] == res[k]["tau"]
+        assert np.array_equal(s.get_state(), Ub[k]), k
+    y.close()
