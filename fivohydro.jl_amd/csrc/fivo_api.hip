@@ -401,6 +401,29 @@ int fivo_create(const fivo_grid_desc* g, const fivo_model_desc* m, fivo_ctx** ou
         return fail(nullptr, FIVO_ERR_ARG, "consistent_m2 requires enable_diff");
     if (m->enable_diff && !m->with_charge)
         return fail(nullptr, FIVO_ERR_ARG, "enable_diff requires with_charge");
+    {
+        // check_terms (src/terms.jl:272-283): a switch changed from its default in
+        // a disabled sector would act on nothing
+        static const char* const names[18] = {
+            "nu_gradalpha", "fm_gradT", "fm_inertial", "fm_nu_gradu", "fm_expansion", "fm_dlnh",
+            "m2_nu_gradient", "m2_bg_gradu", "m2_bg_DlnT", "m2_bg_Dalpha", "m2_expansion", "m2_pi_sigma",
+            "m2_PiQ_sigma", "m2_vorticity", "m2_projector", "m2_accel_nu", "m2_nu_gradTh", "shear_vorticity"};
+        const unsigned changed = (m->terms ^ FIVO_TERMS_DEFAULT) & 0x3FFFFu;
+        if (m->terms & ~0x3FFFFu) return fail(nullptr, FIVO_ERR_ARG, "terms: unknown bits set");
+        for (int k = 0; k < 18; ++k) {
+            if (!((changed >> k) & 1u)) continue;
+            const bool on = k == 0 ? m->enable_diff != 0
+                          : k <= 5 ? m->consistent_fm != 0
+                          : k <= 16 ? m->consistent_m2 != 0
+                                    : m->enable_shear != 0;
+            if (on) continue;
+            const char* sector = k == 0 ? "enable_diff" : k <= 5 ? "consistent_fm" : k <= 16 ? "consistent_m2"
+                                                                                        : "enable_shear";
+            return fail(nullptr, FIVO_ERR_ARG,
+                        std::string("terms: `") + names[k] + "` changes a term of the `" + sector +
+                            "` sector, but `" + sector + " = false`, so the switch would act on nothing");
+        }
+    }
     Ctx* c = new Ctx();
     c->gd = *g;
     c->md = *m;

@@ -7,7 +7,8 @@ reference's Julia API over that ABI.
 from ._abi import (FIVO_BC_OUTFLOW, FIVO_BC_PERIODIC, FIVO_SSPRK2, FIVO_SSPRK3, Lib, load_hip,
                    HIP_LIB_PATH)
 from .model import (ConformalHQEOS, LatticeHRGEOS, Grid2D, make_grid2d, make_grid_2d,
-                    make_layout2d, build_model_2d, allocate_state, resolve_terms, T_MIN, E_FLOOR,
+                    make_layout2d, build_model_2d, allocate_state, resolve_terms, without, check_terms,
+                    TERM_REGISTER, T_MIN, E_FLOOR,
                     HBARC)
 from .solver import Solver, FivoError
 from .batch import Ensemble
@@ -16,5 +17,5 @@ from . import ic
 
 __all__ = ["FIVO_BC_OUTFLOW", "FIVO_BC_PERIODIC", "FIVO_SSPRK2", "FIVO_SSPRK3", "Lib", "load_hip",
            "HIP_LIB_PATH", "ConformalHQEOS", "LatticeHRGEOS", "Grid2D", "make_grid2d",
-           "make_grid_2d", "make_layout2d", "build_model_2d", "allocate_state", "resolve_terms",
+           "make_grid_2d", "make_layout2d", "build_model_2d", "allocate_state", "resolve_terms", "without", "check_terms", "TERM_REGISTER",
            "Solver", "FivoError", "Ensemble", "Mlp", "ic", "T_MIN", "E_FLOOR", "HBARC"]

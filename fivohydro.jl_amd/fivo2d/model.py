@@ -198,22 +198,129 @@ class IdealDiffVisc2DModel:
         return m
 
 
-def resolve_terms(terms="default", **overrides):
-    """Bitmask of src/terms.jl `Terms`: 'default' preset plus per-name overrides."""
-    if isinstance(terms, int):
-        mask = terms
-    elif terms in ("default", None):
-        mask = _abi.TERMS_DEFAULT
-    elif terms == "none":
-        mask = 0
-    elif terms == "full":
-        mask = 0x3FFFF
-    else:
-        raise ValueError(f"unknown terms preset {terms!r}")
-    for k, v in overrides.items():
-        b = 1 << _abi.TERM_NAMES.index(k)
-        mask = (mask | b) if v else (mask & ~b)
-    return mask
+# ---- per-term switches (src/terms.jl) ------------------------------------------------
+# (name, sector, ingredients) of every term, in the order of the Terms fields
+# (src/terms.jl:114-133); a term's bit in the mask is its index here.
+TERM_REGISTER = (
+    ("nu_gradalpha", "enable_diff", ("fugacity_gradient",)),
+    ("fm_gradT", "consistent_fm", ("temperature_gradient",)),
+    ("fm_inertial", "consistent_fm", ("acceleration",)),
+    ("fm_nu_gradu", "consistent_fm", ("velocity_gradient",)),
+    ("fm_expansion", "consistent_fm", ("expansion",)),
+    ("fm_dlnh", "consistent_fm", ("cooling",)),
+    ("m2_nu_gradient", "consistent_m2", ("current_gradient",)),
+    ("m2_bg_gradu", "consistent_m2", ("shear", "expansion")),
+    ("m2_bg_DlnT", "consistent_m2", ("cooling",)),
+    ("m2_bg_Dalpha", "consistent_m2", ("fugacity_rate",)),
+    ("m2_expansion", "consistent_m2", ("expansion", "cooling")),
+    ("m2_pi_sigma", "consistent_m2", ("shear",)),
+    ("m2_PiQ_sigma", "consistent_m2", ("shear",)),
+    ("m2_vorticity", "consistent_m2", ("vorticity",)),
+    ("m2_projector", "consistent_m2", ("acceleration",)),
+    ("m2_accel_nu", "consistent_m2", ("acceleration",)),
+    ("m2_nu_gradTh", "consistent_m2", ("temperature_gradient",)),
+    ("shear_vorticity", "enable_shear", ("vorticity",)),
+)
+INGREDIENTS = ("fugacity_gradient", "temperature_gradient", "acceleration", "shear", "expansion", "vorticity",
+               "velocity_gradient", "cooling", "fugacity_rate", "current_gradient")
+INGREDIENT_GROUPS = {
+    "inertia": ("acceleration",),
+    "velocity_gradient": ("velocity_gradient", "shear", "expansion", "vorticity"),
+    "medium_gradients": ("temperature_gradient", "acceleration", "velocity_gradient", "shear", "expansion",
+                         "vorticity", "cooling"),
+}
+TERM_PRESETS = ("default", "full", "homogeneous", "none")
+
+
+def _term_default(k):
+    return bool((_abi.TERMS_DEFAULT >> k) & 1)
+
+
+def _expand_ingredients(xs):
+    out = []
+    for x in ((xs,) if isinstance(xs, str) else xs):
+        x = x.lstrip(":")
+        if x in INGREDIENT_GROUPS:
+            out += [y for y in INGREDIENT_GROUPS[x] if y not in out]
+        elif x in INGREDIENTS:
+            if x not in out:
+                out.append(x)
+        else:
+            raise ValueError(f"Terms: unknown ingredient :{x}. Valid: " + ", ".join(":" + i for i in INGREDIENTS)
+                             + "; groups: " + ", ".join(":" + g for g in INGREDIENT_GROUPS))
+    return out
+
+
+def _preset_values(p):
+    """terms_preset (src/terms.jl:184-197)."""
+    p = p.lstrip(":")
+    n = len(TERM_REGISTER)
+    if p == "default":
+        return [_term_default(k) for k in range(n)]
+    if p == "full":
+        return [True] * n
+    if p == "none":
+        return [False] * n
+    if p == "homogeneous":
+        ingr = _expand_ingredients(("medium_gradients",))
+        return [_term_default(k) and not any(i in ingr for i in TERM_REGISTER[k][2]) for k in range(n)]
+    raise ValueError(f"Terms: unknown preset :{p}. Valid: " + ", ".join(":" + q for q in TERM_PRESETS))
+
+
+def without(*ingredients, from_="default"):
+    """without(ingredients...; from = :default) (src/terms.jl:199-209): a terms spec
+    that drops every term built from any of `ingredients`."""
+    return {"preset": from_, "without": tuple(ingredients)}
+
+
+def resolve_terms(spec="default", sector_on=None, **overrides):
+    """resolve_terms (src/terms.jl:211-262): the bit mask of the `Terms` switches
+    for a spec: an int mask (taken as is), a preset name ('default', 'full',
+    'homogeneous', 'none'), or a dict with optional 'preset', 'without', 'with'
+    and term names (applied in that order, explicit names last). A change that
+    came from a preset or an ingredient in a disabled sector is reset to the
+    default; an explicit name is kept, so that check_terms refuses it."""
+    if isinstance(spec, int):
+        return spec
+    if spec is None:
+        spec = "default"
+    nt = dict(spec) if isinstance(spec, dict) else {"preset": spec}
+    nt.update(overrides)
+    names = [t[0] for t in TERM_REGISTER]
+    ctrl = ("preset", "without", "with")
+    bad = [k for k in nt if k not in names and k not in ctrl]
+    if bad:
+        raise ValueError(f"Terms: unknown term(s) {bad}. Valid names: " + ", ".join(names)
+                         + " — plus `preset`, `without`, `with`.")
+    vals = _preset_values(nt.get("preset", "default"))
+    for key, on in (("without", False), ("with", True)):
+        if key in nt:
+            ingr = _expand_ingredients(nt[key])
+            for k, (_, _, ing) in enumerate(TERM_REGISTER):
+                if any(i in ingr for i in ing):
+                    vals[k] = on
+    if sector_on is not None:
+        for k, (_, sector, _) in enumerate(TERM_REGISTER):
+            if not sector_on(sector):
+                vals[k] = _term_default(k)
+    for key, v in nt.items():
+        if key in ctrl:
+            continue
+        if not isinstance(v, bool):
+            raise ValueError(f"Terms: `{key}` must be a Bool, got {v!r}")
+        vals[names.index(key)] = v
+    return sum(1 << k for k, v in enumerate(vals) if v)
+
+
+def check_terms(mask, sector_on, solver="build_model_2d"):
+    """check_terms (src/terms.jl:272-283): refuse a switch whose sector is disabled."""
+    for k, (name, sector, _) in enumerate(TERM_REGISTER):
+        v = bool((mask >> k) & 1)
+        if v == _term_default(k) or sector_on(sector):
+            continue
+        raise ValueError(f"{solver}: `{name} = {str(v).lower()}` changes a term of the `{sector}` sector, but "
+                         f"`{sector} = false`, so the switch would act on nothing. Enable `{sector}`, or leave "
+                         f"`{name}` at its default ({str(_term_default(k)).lower()}).")
 
 
 def build_model_2d(eos=None, r_domain=math.inf, enable_shear=False, enable_bulk=False,
@@ -265,10 +372,11 @@ def build_model_2d(eos=None, r_domain=math.inf, enable_shear=False, enable_bulk=
         raise ValueError("consistent_m2 requires enable_diff (src2d/dissipation2d.jl:339-342).")
     layout = make_layout2d(with_charge=with_charge, with_bulk=enable_bulk,
                            with_shear=enable_shear, with_m2=p["consistent_m2"])
-    if isinstance(terms, dict):
-        tmask = resolve_terms("default", **terms)
-    else:
-        tmask = resolve_terms(terms)
+    # terms2d / sector_on_2d (main2D.jl:117-121): the sectors this model carries
+    sectors = {"enable_diff": bool(enable_diff), "enable_shear": bool(enable_shear),
+               "consistent_fm": bool(p["consistent_fm"]), "consistent_m2": bool(p["consistent_m2"])}
+    tmask = resolve_terms(terms, sector_on=sectors.get)
+    check_terms(tmask, sectors.get)
     return IdealDiffVisc2DModel(eos=eos, layout=layout, params=p, terms=tmask)
 
 
