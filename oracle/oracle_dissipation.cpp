@@ -590,3 +590,13 @@ void relax_dissipative(Ctx& c, double tau, double D) {
 }
 
 }  // namespace orc
+
+// single-value probes of the charge-sector coefficients (test infrastructure)
+extern "C" {
+double orc_diff_tauN(const fivo_eos_desc* e, double T, double alpha, double DsT, double tauN_coeff) {
+    return orc::diff_tauN(orc::jmax(T, orc::T_MIN), alpha, orc::EOS(*e), DsT, tauN_coeff);
+}
+void orc_hq_h_hprime(const fivo_eos_desc* e, double T, double* out2) {
+    orc::hq_h_hprime(T, orc::EOS(*e), &out2[0], &out2[1]);
+}
+}
