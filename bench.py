@@ -41,10 +41,10 @@ def log(msg):
 # FETCH_SIZE / WRITE_SIZE, scaled by the same counters on a device copy of known
 # size (profiles/full_sectors_512_pmc_traffic.md); keyed by (config, kernel).
 TRAFFIC = {
-    ("c3", "k_prim_coop"): (2.0 * 1010.4 + 742.3) * 1024,
-    ("c3", "k_prim"): (2.0 * 15255.2 + 33697.1) * 1024,
-    ("c3", "k_faces"): (2.0 * 41842.8 + 34183.4) * 1024,
-    ("c3", "k_relax"): (2.0 * 36540.1 + 56558.3) * 1024,
+    ("c3", "k_prim_coop"): (2.0 * 1010.0 + 739.9) * 1024,
+    ("c3", "k_prim"): (2.0 * 15255.5 + 33696.9) * 1024,
+    ("c3", "k_faces"): (2.0 * 42036.5 + 34297.5) * 1024,
+    ("c3", "k_relax"): (2.0 * 43614.3 + 65265.6) * 1024,
 }
 
 # Workloads: (description, builder). Algorithmic HBM bytes per cell-update and per
