@@ -39,7 +39,7 @@ def log(msg):
 
 # HBM bytes per launch of the dominant kernel measured with rocprofv3 --pmc
 # FETCH_SIZE / WRITE_SIZE, scaled by the same counters on a device copy of known
-# size (profiles/full_sectors_512_pmc_traffic.md); keyed by (config, kernel).
+# size (profiles/full_sectors_512_kernel_stats.md); keyed by (config, kernel).
 TRAFFIC = {
     ("c3", "k_prim_coop"): (2.0 * 1010.0 + 739.9) * 1024,
     ("c3", "k_prim"): (2.0 * 15255.5 + 33696.9) * 1024,
