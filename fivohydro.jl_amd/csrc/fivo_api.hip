@@ -143,6 +143,26 @@ int do_faces(Ctx& c, const FaceLaunch& f) {
     HIPCHK(&c, launch_faces(c, f));
     return FIVO_OK;
 }
+// The primitive pass and the fused faces of one RHS. The face tiles that read no
+// cell queued for the cooperative kernel do not wait for it: they run on a second
+// stream beside it, the rest after it. Per-kernel timing serialises the two.
+int do_prim_faces(Ctx& c, const double* U, double tau, const FaceLaunch& f) {
+    if (c.timing) {
+        const int rc = do_prim(c, U, tau);
+        return rc < 0 ? rc : do_faces(c, f);
+    }
+    HIPCHK(&c, launch_prim_warm(c, U, tau));
+    HIPCHK(&c, launch_mark_tiles(c));
+    HIPCHK(&c, hipEventRecord(c.ev_fork, c.stream));
+    HIPCHK(&c, hipStreamWaitEvent(c.stream2, c.ev_fork, 0));
+    HIPCHK(&c, launch_faces(c, f, 1, c.stream2));
+    HIPCHK(&c, hipEventRecord(c.ev_join, c.stream2));
+    HIPCHK(&c, launch_prim_seed(c, U, tau));
+    HIPCHK(&c, launch_prim_coop(c, U, tau));
+    HIPCHK(&c, hipStreamWaitEvent(c.stream, c.ev_join, 0));
+    HIPCHK(&c, launch_faces(c, f, 2, c.stream));
+    return FIVO_OK;
+}
 
 // rhs_2d! (src2d/rhs2d.jl:169-249) into c.dU
 int rhs_dev(Ctx& c, double tau, int bc, const uint8_t* force_dev) {
@@ -150,11 +170,10 @@ int rhs_dev(Ctx& c, double tau, int bc, const uint8_t* force_dev) {
     int rc;
     if ((rc = reset_scalars(c)) < 0) return rc;
     if ((rc = do_bc(c, U, bc)) < 0) return rc;
-    if ((rc = do_prim(c, U, tau)) < 0) return rc;
     HIPCHK(&c, hipMemsetAsync(c.dU, 0, sizeof(double) * (size_t)c.p.nv * c.p.pitch, c.stream));
     FaceLaunch f{};
     f.Ub = U; f.Uold = U; f.Uout = nullptr; f.dUout = c.dU; f.mode = 0; f.tau = tau; f.force = force_dev;
-    if ((rc = do_faces(c, f)) < 0) return rc;
+    if ((rc = do_prim_faces(c, U, tau, f)) < 0) return rc;
     c.nrhs += 1;
     return read_scalars(c);
 }
@@ -167,11 +186,10 @@ int stage(Ctx& c, double* Ub, const double* Uold, double* Uout, double tau, doub
     for (int attempt = 0; attempt <= max_mood; ++attempt) {
         if ((rc = reset_scalars(c)) < 0) return rc;
         if (bc_first && attempt == 0 && (rc = do_bc(c, Ub, bc)) < 0) return rc;
-        if ((rc = do_prim(c, Ub, tau)) < 0) return rc;
         FaceLaunch f{};
         f.Ub = Ub; f.Uold = Uold; f.Uout = Uout; f.dUout = nullptr; f.mode = 1; f.combine = combine;
         f.tau = tau; f.tau_state = tau_state; f.D = D; f.wold = wold; f.wnew = wnew; f.force = mask;
-        if ((rc = do_faces(c, f)) < 0) return rc;
+        if ((rc = do_prim_faces(c, Ub, tau, f)) < 0) return rc;
         if ((rc = do_bc(c, Uout, bc)) < 0) return rc;
         c.nrhs += 1;
         if ((rc = read_scalars(c)) < 0) return rc;
@@ -474,7 +492,8 @@ int fivo_create(const fivo_grid_desc* g, const fivo_model_desc* m, fivo_ctx** ou
     }
     const size_t P = p.pitch;
     const size_t ndbl = 4 * nv * P + nv * P + nv * P + 24 * P + 3 * (size_t)c->npartials + 64;
-    const size_t nbyte = ndbl * sizeof(double) + 4 * P + sizeof(PrimJob) * P + 256;
+    c->ntiles_max = ((p.Nx + 6) / 7) * ((p.Ny + 7) / 8);   // the smallest face tile, 7 x 8
+    const size_t nbyte = ndbl * sizeof(double) + 4 * P + sizeof(PrimJob) * P + (size_t)c->ntiles_max + 256;
     c->pool_bytes = nbyte;
     if ((e = hipMalloc(&c->pool, nbyte)) != hipSuccess) {
         (void)hipStreamDestroy(c->stream); delete c;
@@ -497,9 +516,13 @@ int fivo_create(const fivo_grid_desc* g, const fivo_model_desc* m, fivo_ctx** ou
     c->w.force = b; b += P;
     c->w.ftmp = b; b += P;
     c->jobs = reinterpret_cast<PrimJob*>(b); b += sizeof(PrimJob) * P;
+    c->dirty = b; b += c->ntiles_max;
     if ((e = hipMalloc(&c->sc, sizeof(DevScalars))) != hipSuccess ||
         (e = hipHostMalloc(&c->sc_host, sizeof(DevScalars), hipHostMallocDefault)) != hipSuccess ||
-        (e = hipEventCreate(&c->ev[0])) != hipSuccess || (e = hipEventCreate(&c->ev[1])) != hipSuccess) {
+        (e = hipEventCreate(&c->ev[0])) != hipSuccess || (e = hipEventCreate(&c->ev[1])) != hipSuccess ||
+        (e = hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming)) != hipSuccess) {
         fivo_destroy(reinterpret_cast<fivo_ctx*>(c));
         return fail(nullptr, FIVO_ERR_HIP, hipGetErrorString(e));
     }
@@ -525,6 +548,9 @@ void fivo_destroy(fivo_ctx* ctx) {
     if (c->sc_host) (void)hipHostFree(c->sc_host);
     for (auto& ev : c->ev) if (ev) (void)hipEventDestroy(ev);
     for (auto& ev : c->evpool) (void)hipEventDestroy(ev);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->stream2) { (void)hipStreamSynchronize(c->stream2); (void)hipStreamDestroy(c->stream2); }
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }

@@ -153,6 +153,8 @@ __global__ __launch_bounds__((face_threads<TX, TY>())) void kb_faces(BatchRun R,
     K.tiles_y = E->tiles_y;
     K.nmain = E->nmain;
     K.remap = 0;
+    K.pass = 0;           // every tile: inside a captured step the kernels run in stream order
+    K.dirty = nullptr;
     const int cur = c->cur;
     const int s = phase == PH_INIT ? 1 : phase;
     FaceLaunch& f = K.f;

@@ -396,6 +396,11 @@ struct FaceK {
     DevScalars* sc;
     int tiles_y, nmain;
     int remap;            // XCD-aware tile order (one grid per event)
+    // 0: every tile; 1: the tiles no queued cell of the primitive pass touches (run
+    // beside the cooperative kernel); 2: the marked tiles and the ghost-region faces
+    // (after it), clearing the marks
+    int pass;
+    uint8_t* dirty;       // one mark per tile
 };
 // main tiles + extra blocks for the ghost-region faces of a TX x TY tiling
 // Threads of a face-tile block: the tile's cells rounded up to whole wavefronts.
@@ -404,6 +409,23 @@ struct FaceK {
 // rounds, where a 256-cell tile needs a third round for its TX + TY halo faces.
 template <int TX, int TY>
 constexpr int face_threads() { return (TX * TY + 63) / 64 * 64; }
+
+// floor(a / b) for b > 0 and any sign of a
+__device__ __forceinline__ int floordiv(int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
+
+// Mark the face tiles whose work reads the primitives of cell i: a tile's faces
+// read the cells [x0 - 2, x0 + TX + 1] x [y0 - 2, y0 + TY + 1] at most (MUSCL
+// stencils of its halo faces), and its cell update its own cells.
+template <int TX, int TY>
+__device__ __forceinline__ void mark_tiles(int Nytot, int ng, int i, uint8_t* dirty, int tiles_y, int nmain) {
+    const int gx = i / Nytot;
+    const int ix = gx - ng, iy = i - gx * Nytot - ng;   // interior-origin coordinates
+    const int tiles_x = nmain / tiles_y;
+    const int alo = max(floordiv(ix - 2, TX), 0), ahi = min(floordiv(ix + 2, TX), tiles_x - 1);
+    const int blo = max(floordiv(iy - 2, TY), 0), bhi = min(floordiv(iy + 2, TY), tiles_y - 1);
+    for (int a = alo; a <= ahi; ++a)
+        for (int b = blo; b <= bhi; ++b) dirty[a * tiles_y + b] = 1;
+}
 
 template <int TX, int TY>
 __host__ __device__ inline void face_grid(const Params& p, int* tiles_y, int* nmain, int* nblocks) {
@@ -623,6 +645,7 @@ __device__ __forceinline__ void faces_body(const FaceK& K, const int bid, double
     double amax = 0.0;
     if (bid < K.nmain) {
         const int b = K.remap ? xcd_remap(bid, K.nmain) : bid;
+        if (K.pass != 0 && ((K.dirty[b] != 0) != (K.pass == 2))) return;   // the other pass's tile
         const int tx_t = b / K.tiles_y, ty_t = b - tx_t * K.tiles_y;
         const int x0 = p.ng + tx_t * TX, y0 = p.ng + ty_t * TY;
         const int nxt = min(TX, p.ng + p.Nx - x0), nyt = min(TY, p.ng + p.Ny - y0);
@@ -647,6 +670,7 @@ __device__ __forceinline__ void faces_body(const FaceK& K, const int bid, double
             }
         }
         __syncthreads();
+        if (K.pass == 2 && tid == 0) K.dirty[b] = 0;   // every thread read the mark before the barrier
         // ---- per cell: divergence + source, then the RK stage update ----
         const int cx = tid / TY, cy = tid - cx * TY;
         if (cx < nxt && cy < nyt) {
@@ -712,6 +736,7 @@ __device__ __forceinline__ void faces_body(const FaceK& K, const int bid, double
             }
         }
     } else {
+        if (K.pass == 1) return;   // ghost cells may be queued: these wait for every primitive
         // faces outside every interior cell's stencil: the reference evaluates them
         // too and their signal speeds enter amax (src2d/rhs2d.jl:290-367)
         const int ng = p.ng, Nx = p.Nx, Ny = p.Ny, Nxt = p.Nxtot, Nyt = p.Nytot;

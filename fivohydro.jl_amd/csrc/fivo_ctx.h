@@ -90,6 +90,10 @@ struct Ctx {
     double* snap = nullptr;         // pre-relaxation snapshot (dissipation2d.jl:299-303)
     double* partials = nullptr;     // reduction partials
     PrimJob* jobs = nullptr;        // cells of a primitive pass queued for the cooperative solve
+    uint8_t* dirty = nullptr;       // face tiles touching a queued cell (one byte per tile of the smallest tile)
+    int ntiles_max = 0;
+    hipStream_t stream2 = nullptr;  // the face tiles that run beside the cooperative kernel
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     int npartials = 0;
     DevWork w;
     DevScalars* sc = nullptr;       // device
@@ -130,7 +134,9 @@ hipError_t launch_prim(Ctx& c, const double* U, double tau);        // the three
 hipError_t launch_prim_warm(Ctx& c, const double* U, double tau);
 hipError_t launch_prim_seed(Ctx& c, const double* U, double tau);   // no-op without a learned seed
 hipError_t launch_prim_coop(Ctx& c, const double* U, double tau);
-hipError_t launch_faces(Ctx& c, const FaceLaunch& f);
+// pass 0: every tile; 1: the tiles no queued cell touches; 2: the rest (see FaceK)
+hipError_t launch_faces(Ctx& c, const FaceLaunch& f, int pass = 0, hipStream_t st = nullptr);
+hipError_t launch_mark_tiles(Ctx& c);   // marks from the queue of the last warm primitive pass
 hipError_t launch_expand(Ctx& c, const uint8_t* in, uint8_t* out);
 hipError_t launch_sanitize(Ctx& c, double* U, double tau, int use_P);
 hipError_t launch_admissible(Ctx& c, const double* U, double tau, int use_work, uint8_t* bad_out);

@@ -265,10 +265,10 @@ hipError_t launch_prim(Ctx& c, const double* U, double tau) {
 }
 
 template <int EK, bool NCH, int TX, int TY>
-static hipError_t faces_tx(Ctx& c, const FaceLaunch& f) {
+static hipError_t faces_tx(Ctx& c, const FaceLaunch& f, int pass, hipStream_t st) {
     const Params& p = c.p;
     FaceK K;
-    K.p = p; K.f = f; K.w = c.w; K.sc = c.sc; K.remap = 1;
+    K.p = p; K.f = f; K.w = c.w; K.sc = c.sc; K.remap = 1; K.pass = pass; K.dirty = c.dirty;
     int nb;
     face_grid<TX, TY>(p, &K.tiles_y, &K.nmain, &nb);
     constexpr int NF = (TX + 1) * TY + TX * (TY + 1);
@@ -284,18 +284,39 @@ static hipError_t faces_tx(Ctx& c, const FaceLaunch& f) {
         if (e != hipSuccess) return e;
         lds_set = lds;
     }
-    if (f.mode == 0) hipLaunchKernelGGL((k_faces<EK, NCH, TX, TY, 0>), dim3(nb), dim3(face_threads<TX, TY>()), lds, c.stream, K);
-    else hipLaunchKernelGGL((k_faces<EK, NCH, TX, TY, 1>), dim3(nb), dim3(face_threads<TX, TY>()), lds, c.stream, K);
+    if (f.mode == 0) hipLaunchKernelGGL((k_faces<EK, NCH, TX, TY, 0>), dim3(nb), dim3(face_threads<TX, TY>()), lds, st, K);
+    else hipLaunchKernelGGL((k_faces<EK, NCH, TX, TY, 1>), dim3(nb), dim3(face_threads<TX, TY>()), lds, st, K);
     return hipGetLastError();
 }
 
-hipError_t launch_faces(Ctx& c, const FaceLaunch& f) {
+hipError_t launch_faces(Ctx& c, const FaceLaunch& f, int pass, hipStream_t st) {
     hipError_t e = hipSuccess;
+    if (!st) st = c.stream;
     // 15 x 16 cells (one 256-lane block, two face rounds) from 300^2 up; 7 x 8 (one
     // wavefront) below, where larger tiles would leave CUs without blocks
-    if (c.tile == 0) EOS_DISPATCH(c, e = (faces_tx<EK, NCH, 15, 16>(c, f)));
-    else EOS_DISPATCH(c, e = (faces_tx<EK, NCH, 7, 8>(c, f)));
+    if (c.tile == 0) EOS_DISPATCH(c, e = (faces_tx<EK, NCH, 15, 16>(c, f, pass, st)));
+    else EOS_DISPATCH(c, e = (faces_tx<EK, NCH, 7, 8>(c, f, pass, st)));
     return e;
+}
+
+template <int TX, int TY>
+__global__ __launch_bounds__(256) void k_mark_tiles(Params p, const DevScalars* __restrict__ sc,
+                                                    const PrimJob* __restrict__ jobs, uint8_t* dirty, int tiles_y,
+                                                    int nmain) {
+    const int n = (int)sc->ncold;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x)
+        mark_tiles<TX, TY>(p.Nytot, p.ng, jobs[t].cell, dirty, tiles_y, nmain);
+}
+template <int TX, int TY>
+static hipError_t mark_tx(Ctx& c) {
+    int tiles_y, nmain, nb;
+    face_grid<TX, TY>(c.p, &tiles_y, &nmain, &nb);
+    hipLaunchKernelGGL((k_mark_tiles<TX, TY>), dim3(64), dim3(256), 0, c.stream, c.p, c.sc, c.jobs, c.dirty, tiles_y,
+                       nmain);
+    return hipGetLastError();
+}
+hipError_t launch_mark_tiles(Ctx& c) {
+    return c.tile == 0 ? mark_tx<15, 16>(c) : mark_tx<7, 8>(c);
 }
 
 hipError_t launch_expand(Ctx& c, const uint8_t* in, uint8_t* out) {

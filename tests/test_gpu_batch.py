@@ -83,6 +83,26 @@ def test_ensemble_equals_one_event_runs(hip, tile):
         assert ra["Q1"] == rb["Q1"] and ra["max_shear_res"] == rb["max_shear_res"]
 
 
+def test_face_overlap_equals_serial_order(hip):
+    """Face tiles that read no queued cell run on a second stream beside the
+    cooperative primitive kernel, the rest after it. Kernel timing serialises the
+    two streams, so a timed run gives the serial order: the bits must not change."""
+    m = full_model()
+    for k, (g, prims) in enumerate(events()):
+        ra, Ua, Wa = run_alone(hip, g, m, prims)
+        s = F.Solver(g, m, lib=hip)
+        assert hip.enable_timing(s.ctx, 1) == 0
+        s.init_from_prims(RUN["tau0"], **prims)
+        rb = s.run_sim_2d(integrator="ssprk3", **RUN)
+        print(f"event {k}: {ra['nqueued']} cells queued over {ra['nsteps']} steps", flush=True)
+        assert ra["nqueued"] > 0 and ra["nqueued"] == rb["nqueued"] and ra["tau"] == rb["tau"]
+        assert np.array_equal(Ua, s.get_state()), k
+        Wb = s.get_work(("yT", "ux", "uy"))
+        for f in ("yT", "ux", "uy"):
+            assert np.array_equal(Wa[f], Wb[f]), (k, f)
+        s.close()
+
+
 def test_ensemble_autotuned_matches_oracle(oracle, hip):
     """Autotuned ensemble vs the CPU oracle per event: final T and u within 1e-9."""
     m = full_model()
