@@ -20,6 +20,9 @@ timed region runs without them).
 cpu_baseline: the project's FP64 C++ restatement of the reference solver (the
 parity oracle), OpenMP over x-columns on the host cores, on a bounded sample of
 the same workload.
+background_export: the readout row (tools/export_background2d.jl) at its
+defaults, end to end on the GPU and on the oracle, and its slice kernel's HBM
+rate on the 512^2 state.
 """
 import argparse
 import ctypes as C
@@ -148,6 +151,51 @@ def train_warm_start(lib):
             "mse_normalised": {k: float(v) for k, v in zip(("yT", "phi", "asinh_us", "u_perp", "log_e", "log_P"), mse)},
             "training_events": "6 events 192^2, 50-250 spots, seeds 7000-7005, 13 primitive passes each"}
     return net, info
+
+
+BG_WORKLOAD = ("export_background2d at its defaults: 200x200 on [-16,16] fm, tau 0.4 -> 8 fm/c in 39 uniform "
+               "slices, SSPRK2, CFL 0.2, CFLtau 0.02, the exporter's medium (LatticeHRG, eta/s 0.1, zeta/s 0.1, "
+               "D_sT 0.1163); a seeded hot-spot event (200 spots, sigma 0.5 fm, peak T 0.45 GeV) stands in for "
+               "the event file the reference does not ship")
+
+
+def _bg_event(g):
+    from fivo2d import ic
+    return ic.hotspot_event(g, seed=20, nspots=200, sigma=0.5, Tpeak=0.45)
+
+
+def background_bench(lib, s, g):
+    """The readout row: export_background2d end to end at its defaults (tables in
+    HBM, one copy to the host at the end), and the slice kernel alone on the 512^2
+    state: 64 algorithmic bytes per interior cell (yT, ux, uy, n read; T, vx, vy, n
+    written), average launch time from HIP events on the library stream."""
+    import torch
+    import fivo2d as F
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = F.export_background2d(_bg_event, lib=lib, verbose=False)
+    wall = time.perf_counter() - t0
+    run = res["_run"]
+    out = {"workload": BG_WORKLOAD, "wall_s": wall, "steps": run["steps"],
+           "cell_updates_per_s": run["steps"] * 200 * 200 / wall, "max_u": run["max_u"], "max_v": run["max_v"]}
+    nt = 40
+    tabs = [torch.empty(nt * g.Nx * g.Ny, dtype=torch.float64, device="cuda") for _ in range(4)]
+    lib.enable_timing(s.ctx, 1)
+    before = C.c_double(0)
+    n0 = lib.kernel_time(s.ctx, 7, C.byref(before))
+    for k in range(nt):
+        s.background_slice(k, nt, *tabs)
+    tot = C.c_double(0)
+    n1 = lib.kernel_time(s.ctx, 7, C.byref(tot))
+    avg_ms = (tot.value - before.value) / (n1 - n0)
+    alg = 64 * g.Nx * g.Ny
+    out["slice_kernel"] = {"kernel": "k_background", "grid": f"{g.Nx}x{g.Ny}", "launches": n1 - n0,
+                           "avg_launch_us": avg_ms * 1e3, "alg_bytes_per_launch": alg,
+                           "achieved_GBs": alg / (avg_ms * 1e-3) / 1e9,
+                           "frac_of_hbm_peak": alg / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    log(f"background export: {wall:.2f} s ({run['steps']} steps); slice kernel {avg_ms * 1e3:.1f} us, "
+        f"{out['slice_kernel']['achieved_GBs']:.0f} GB/s")
+    return out
 
 
 def ensemble_bench(lib, name, steps, warmup, net):
@@ -323,7 +371,7 @@ def run_steps(s, tau, nsteps, cfg, reset):
     return r
 
 
-def cpu_baseline(cfg, budget_s=12.0):
+def cpu_baseline(cfg, budget_s=12.0, background=False):
     import fivo2d as F
     orc = F.Lib(os.path.join(REPO, "oracle", "liboracle2d.so"), "orc_")
     orc.dll.orc_set_threads.restype = C.c_int
@@ -340,10 +388,20 @@ def cpu_baseline(cfg, budget_s=12.0):
     t0 = time.perf_counter()
     r = run_steps(s, r["tau"], n, cfg, False)
     wall = time.perf_counter() - t0
-    return {"value": n * g.Nx * g.Ny / wall, "unit": "cell-updates/s", "cores": threads,
-            "kind": "port",
-            "sample": f"{n} steps of {cfg['workload']} ({g.Nx}x{g.Ny}) after a 2-step warm-up, "
-                      f"{wall:.1f} s wall, OpenMP over x-columns"}
+    out = {"value": n * g.Nx * g.Ny / wall, "unit": "cell-updates/s", "cores": threads,
+           "kind": "port",
+           "sample": f"{n} steps of {cfg['workload']} ({g.Nx}x{g.Ny}) after a 2-step warm-up, "
+                     f"{wall:.1f} s wall, OpenMP over x-columns"}
+    s.close()
+    if background:
+        # the readout row's workload, whole, on the oracle
+        t0 = time.perf_counter()
+        res = F.export_background2d(_bg_event, lib=orc, verbose=False)
+        wb = time.perf_counter() - t0
+        out["background_export"] = {"workload": BG_WORKLOAD, "wall_s": wb, "steps": res["_run"]["steps"],
+                                    "cell_updates_per_s": res["_run"]["steps"] * 200 * 200 / wb,
+                                    "cores": threads}
+    return out
 
 
 def kernel_times(lib, ctx):
@@ -351,7 +409,7 @@ def kernel_times(lib, ctx):
     event pair around each launch on the library stream)."""
     tms = {}
     for cat, name in ((0, "k_prim"), (5, "k_prim_coop"), (6, "k_prim_seed"), (1, "k_faces"), (2, "k_bc"),
-                      (3, "k_relax")):
+                      (3, "k_relax"), (7, "k_background")):
         tot = C.c_double(0)
         cnt = lib.kernel_time(ctx, cat, C.byref(tot))
         if cnt > 0:
@@ -367,8 +425,8 @@ def main():
     ap.add_argument("--config", default="c3", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true", help="plain path only (no network, no ensembles)")
-    ap.add_argument("--legs", default="learned512,c4,c5",
-                    help="comma list of the legs beside the plain one: learned512, c4, c5")
+    ap.add_argument("--legs", default="learned512,c4,c5,background",
+                    help="comma list of the legs beside the plain one: learned512, c4, c5, background")
     ap.add_argument("--ensemble-steps", type=int, default=10)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a device copy of 512 MiB (the reference kernel of the traffic counters)")
@@ -436,7 +494,7 @@ def main():
         torch.cuda.synchronize()
         out["copy_calibration_bytes"] = src.numel() * 8
     legs = set() if a.no_ensemble else {x for x in a.legs.split(",") if x}
-    if legs:
+    if legs & {"learned512", "c4", "c5"}:
         net, tinfo = train_warm_start(lib)
         out["learned_warm_start"] = tinfo
         if a.config == "c3" and "learned512" in legs:
@@ -475,9 +533,15 @@ def main():
             if e in legs:
                 out[f"ensemble_{e}"] = ensemble_bench(lib, e, a.ensemble_steps, 3, net)
         net.close()
+    if "background" in legs:
+        out["background_export"] = background_bench(lib, s, g)
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(cfg)
+        out["cpu_baseline"] = cpu_baseline(cfg, background="background" in legs)
         log(f"cpu baseline: {out['cpu_baseline']['value']:.3e} cell-updates/s")
+        if "background_export" in out["cpu_baseline"]:
+            cb = out["cpu_baseline"]["background_export"]
+            out["background_export"]["speedup_vs_cpu_oracle"] = cb["wall_s"] / out["background_export"]["wall_s"]
+            log(f"cpu background export: {cb['wall_s']:.1f} s")
     print(json.dumps(out))
 
 

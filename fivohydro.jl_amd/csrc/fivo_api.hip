@@ -722,6 +722,39 @@ int fivo_update_primitives(fivo_ctx* ctx, double tau, int bc, fivo_counts* cnt) 
     return FIVO_OK;
 }
 
+int fivo_background_slice(fivo_ctx* ctx, int32_t k, int32_t nt, double* T, double* vx, double* vy, double* n,
+                          int32_t device, double* maxima) {
+    Ctx& c = *reinterpret_cast<Ctx*>(ctx);
+    if (!T || !vx || !vy || nt < 1 || k < 0 || k >= nt)
+        return fail(&c, FIVO_ERR_ARG, "background_slice: need T, vx, vy and 0 <= k < nt");
+    const int64_t cells = (int64_t)c.p.Nx * c.p.Ny;
+    HIPCHK(&c, hipMemsetAsync(&c.sc->bg_umax_bits, 0, 2 * sizeof(unsigned long long), c.stream));
+    if (device) {
+        Timer t(c, TK_BG);
+        HIPCHK(&c, launch_background(c, cells * k, T, vx, vy, n));
+    } else {
+        // host tables: the slice is staged in the snapshot buffer (nv >= 4 rows of Ntot)
+        double* st = c.snap;
+        {
+            Timer t(c, TK_BG);
+            HIPCHK(&c, launch_background(c, 0, st, st + cells, st + 2 * cells, n ? st + 3 * cells : nullptr));
+        }
+        double* dst[4] = {T, vx, vy, n};
+        for (int f = 0; f < 4; ++f)
+            if (dst[f])
+                HIPCHK(&c, hipMemcpyAsync(dst[f] + cells * k, st + cells * f, sizeof(double) * cells,
+                                          hipMemcpyDeviceToHost, c.stream));
+    }
+    HIPCHK(&c, hipMemcpyAsync(c.sc_host, c.sc, sizeof(DevScalars), hipMemcpyDeviceToHost, c.stream));
+    HIPCHK(&c, hipStreamSynchronize(c.stream));
+    resolve_timing(c);
+    if (maxima) {
+        const unsigned long long b[2] = {c.sc_host->bg_umax_bits, c.sc_host->bg_vmax_bits};
+        for (int j = 0; j < 2; ++j) std::memcpy(&maxima[j], &b[j], sizeof(double));
+    }
+    return FIVO_OK;
+}
+
 int fivo_sanitize(fivo_ctx* ctx, double tau, int bc) {
     Ctx& c = *reinterpret_cast<Ctx*>(ctx);
     HIPCHK(&c, launch_sanitize(c, c.U[c.cur], tau, 1));

@@ -54,6 +54,8 @@ struct DevScalars {
     unsigned long long nlearned;    // cumulative: cells started from the learned seed
     unsigned long long nfallback;   // cumulative: of those, cells that fell back to the cold seed
     unsigned long long nqueued;     // cumulative: cells queued for the cooperative kernel
+    unsigned long long bg_umax_bits;// background slice: max |u|, max |v| (double bits; NaN sticks)
+    unsigned long long bg_vmax_bits;
 };
 
 // Per-cell work arrays on the device (Work2D, src2d/work2d.jl:15-100)
@@ -74,7 +76,7 @@ struct DevWork {
 
 // timing categories: warm primitive kernel, fused faces, ghost fill, relaxation,
 // everything, cooperative primitive kernel, learned seeds
-enum { TK_PRIM = 0, TK_FACES = 1, TK_BC = 2, TK_RELAX = 3, TK_ALL = 4, TK_COOP = 5, TK_SEED = 6, TK_N = 7 };
+enum { TK_PRIM = 0, TK_FACES = 1, TK_BC = 2, TK_RELAX = 3, TK_ALL = 4, TK_COOP = 5, TK_SEED = 6, TK_BG = 7, TK_N = 8 };
 
 struct Ctx {
     fivo_grid_desc gd;
@@ -107,8 +109,8 @@ struct Ctx {
     MlpDev mlp{};
     // timing
     bool timing = false;
-    double t_ms[TK_N] = {0, 0, 0, 0, 0, 0, 0};
-    int64_t t_cnt[TK_N] = {0, 0, 0, 0, 0, 0, 0};
+    double t_ms[TK_N] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int64_t t_cnt[TK_N] = {0, 0, 0, 0, 0, 0, 0, 0};
     hipEvent_t ev[2] = {nullptr, nullptr};
     std::vector<hipEvent_t> evpool;
     std::vector<int> ev_cat;
@@ -151,6 +153,9 @@ hipError_t launch_reduce_diag(Ctx& c, const double* U, int what);
 hipError_t launch_project_shear(Ctx& c, double* U, int correct);
 hipError_t launch_relax(Ctx& c, double* U, double tau, double dtau);
 hipError_t launch_layout(Ctx& c, const double* aos_dev, double* soa, int to_soa);
+// one slice of a transport background into column-major (Nx, Ny, *) tables at
+// element offset `off` (n may be null); maxima into sc->bg_{u,v}max_bits
+hipError_t launch_background(Ctx& c, int64_t off, double* T, double* vx, double* vy, double* n);
 
 // ---- ensembles (fivo_batch.hip, fivo_batch_kernels.hip) -----------------------
 // An ensemble is a set of events, each a one-event context (its own grid, state

@@ -384,6 +384,89 @@ hipError_t launch_project_shear(Ctx& c, double* U, int correct) {
     return hipGetLastError();
 }
 
+// ---- readout: one time slice of a transport background -------------------------------
+// export_background2d (tools/export_background2d.jl:79-94): T = exp(yT) and the
+// VELOCITY v^i = u^i / u^tau (u^tau = sqrt(1 + u^2)) and n of every interior cell,
+// from the primitives of the last pass, into column-major (Nx, Ny, nt) tables
+// (element (i, j, k) at i + Nx (j + Ny k)). The grid is y-fastest and the tables
+// x-fastest: a 16 x 16 tile (one cell per thread) is read along y into LDS and
+// written along x, so both sides move whole 128-byte lines. max |u| and max |v|
+// go to the scalars; a NaN sticks (its bits sort above +inf), as Julia's max
+// propagates it.
+constexpr int BG_TILE = 16;
+__device__ __forceinline__ unsigned long long bg_key(double h) {   // h >= 0 or NaN
+    return h != h ? 0x7ff8000000000000ull : (unsigned long long)__double_as_longlong(h);
+}
+__global__ __launch_bounds__(BG_TILE * BG_TILE) void k_background(Params p, DevWork w, int64_t off,
+                                                                  double* __restrict__ T, double* __restrict__ vx,
+                                                                  double* __restrict__ vy, double* __restrict__ n,
+                                                                  DevScalars* sc) {
+    __shared__ double s[4][BG_TILE][BG_TILE + 1];
+    const int tiles_y = (p.Ny + BG_TILE - 1) / BG_TILE;
+    const int bx = blockIdx.x / tiles_y, by = blockIdx.x - bx * tiles_y;
+    const int x0 = bx * BG_TILE, y0 = by * BG_TILE;
+    const int tid = threadIdx.x;
+    unsigned long long ku = 0, kv = 0;
+    {
+        const int xl = tid / BG_TILE, yl = tid % BG_TILE;
+        const int x = x0 + xl, y = y0 + yl;
+        if (x < p.Nx && y < p.Ny) {
+            const int l = (x + p.ng) * p.Nytot + y + p.ng;
+            const double ux = w.ux[l], uy = w.uy[l];
+            // separate statements: no multiply-add fusion, as the reference rounds
+            const double uxx = ux * ux;
+            const double uyy = uy * uy;
+            const double ut = sqrt(1.0 + uxx + uyy);
+            const double ax = ux / ut, ay = uy / ut;
+            s[0][xl][yl] = exp(w.yT[l]);
+            s[1][xl][yl] = ax;
+            s[2][xl][yl] = ay;
+            s[3][xl][yl] = w.n[l];
+            ku = bg_key(hypot(ux, uy));
+            kv = bg_key(hypot(ax, ay));
+        }
+    }
+    __syncthreads();
+    {
+        const int yl = tid / BG_TILE, xl = tid % BG_TILE;
+        const int x = x0 + xl, y = y0 + yl;
+        if (x < p.Nx && y < p.Ny) {
+            const int64_t o = off + x + (int64_t)p.Nx * y;
+            T[o] = s[0][xl][yl];
+            vx[o] = s[1][xl][yl];
+            vy[o] = s[2][xl][yl];
+            if (n) n[o] = s[3][xl][yl];
+        }
+    }
+    // one atomic per block at most, and none once the block's maximum is not above
+    // what the scalars already hold (they only grow, so a stale read is safe):
+    // thousands of atomics on one address would cost ten times the slice itself
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long a = __shfl_xor(ku, o, 64), b = __shfl_xor(kv, o, 64);
+        ku = a > ku ? a : ku;
+        kv = b > kv ? b : kv;
+    }
+    __shared__ unsigned long long red[2][BG_TILE * BG_TILE / 64];
+    if ((tid & 63) == 0) { red[0][tid >> 6] = ku; red[1][tid >> 6] = kv; }
+    __syncthreads();
+    if (tid == 0) {
+        for (int j = 1; j < BG_TILE * BG_TILE / 64; ++j) {
+            ku = red[0][j] > ku ? red[0][j] : ku;
+            kv = red[1][j] > kv ? red[1][j] : kv;
+        }
+        if (ku > __atomic_load_n(&sc->bg_umax_bits, __ATOMIC_RELAXED)) atomicMax(&sc->bg_umax_bits, ku);
+        if (kv > __atomic_load_n(&sc->bg_vmax_bits, __ATOMIC_RELAXED)) atomicMax(&sc->bg_vmax_bits, kv);
+    }
+}
+
+hipError_t launch_background(Ctx& c, int64_t off, double* T, double* vx, double* vy, double* n) {
+    const int nb = ((c.p.Nx + BG_TILE - 1) / BG_TILE) * ((c.p.Ny + BG_TILE - 1) / BG_TILE);
+    hipLaunchKernelGGL(k_background, dim3(nb), dim3(BG_TILE * BG_TILE), 0, c.stream, c.p, c.w, off, T, vx, vy, n,
+                       c.sc);
+    return hipGetLastError();
+}
+
 hipError_t launch_layout(Ctx& c, const double* src, double* dst, int to_soa) {
     hipLaunchKernelGGL(k_layout, dim3(nblk((int64_t)c.p.Ntot * c.p.nv, 256)), dim3(256), 0, c.stream, src, dst,
                        c.p, to_soa);

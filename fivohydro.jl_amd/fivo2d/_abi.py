@@ -127,6 +127,7 @@ ABI_FUNCS = (
     ("apply_bc", C.c_int, [_VP, C.c_int]),
     ("rhs", C.c_int, [_VP, C.c_double, C.c_int, C.POINTER(C.c_uint8), _DP, C.POINTER(Counts)]),
     ("update_primitives", C.c_int, [_VP, C.c_double, C.c_int, C.POINTER(Counts)]),
+    ("background_slice", C.c_int, [_VP, C.c_int32, C.c_int32, _DP, _DP, _DP, _DP, C.c_int32, _DP]),
     ("sanitize", C.c_int, [_VP, C.c_double, C.c_int]),
     ("state_ok", C.c_int, [_VP, C.c_double, C.POINTER(C.c_int), C.POINTER(C.c_int64)]),
     ("compute_dt", C.c_int, [_VP, C.c_double, C.c_double, C.c_double, _DP]),

@@ -14,6 +14,7 @@ reference's sweep and driver functions under their own names:
   project_shear        main2D.jl:332-358
   run_sim_2d           main2D.jl:371-561
   fields_2d            main2D.jl:588-600
+  background_slice     tools/export_background2d.jl:79-94 (one time slice)
   set_cell/finalize_ic main2D.jl:142-171
 
 State arrays cross the boundary in the reference layout (U[i, a] here, i.e.
@@ -256,6 +257,45 @@ class Solver:
         return r.as_dict()
 
     # ---- readout -------------------------------------------------------------
+    def background_slice(self, k, nt, T, vx, vy, n=None):
+        """One time slice of export_background2d (tools/export_background2d.jl:79-94)
+        from the primitives of the last primitive pass: T = exp(yT), the velocity
+        v^i = u^i / u^tau and n of every interior cell into element (i, j, k) of
+        (Nx, Ny, nt) tables, column-major (Julia's T_grid[i, j, k]).
+
+        Host tables: numpy float64 arrays of shape (Nx, Ny, nt) in Fortran order.
+        Device tables (the HIP library only): contiguous float64 GPU tensors of
+        Nx*Ny*nt values in the same order, e.g. torch tensors of shape (nt, Ny, Nx);
+        the slice is written in HBM and nothing crosses to the host.
+        Returns (max |u|, max |v|) over the slice (NaN if a cell's is NaN)."""
+        g = self.g
+        shape = (g.Nx, g.Ny, int(nt))
+        tabs = [T, vx, vy, n]
+        on_dev = [hasattr(a, "data_ptr") for a in tabs if a is not None]
+        if any(on_dev) != all(on_dev):
+            raise ValueError("background_slice: the tables must all be on the host or all on the device")
+        device = bool(on_dev and on_dev[0])
+        ptrs = []
+        for a in tabs:
+            if a is None:
+                ptrs.append(None)
+            elif device:
+                if not (a.is_cuda and str(a.dtype) == "torch.float64" and a.is_contiguous()
+                        and a.numel() == g.Nx * g.Ny * int(nt)):
+                    raise ValueError("background_slice: device tables must be contiguous float64 GPU "
+                                     f"tensors of {g.Nx}*{g.Ny}*{int(nt)} values")
+                ptrs.append(C.cast(C.c_void_p(a.data_ptr()), _DP))
+            else:
+                if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.shape == shape
+                        and a.flags.f_contiguous):
+                    raise ValueError(f"background_slice: host tables must be float64 arrays of shape {shape} "
+                                     "in Fortran order")
+                ptrs.append(_dp(a))
+        mx = (C.c_double * 2)()
+        self._chk(self.lib.background_slice(self.ctx, int(k), int(nt), ptrs[0], ptrs[1], ptrs[2], ptrs[3],
+                                            int(device), mx), "background_slice")
+        return mx[0], mx[1]
+
     def fields_2d(self):
         """fields_2d (main2D.jl:588-600): interior fields as (Nx, Ny) arrays [ix, iy]."""
         g = self.g

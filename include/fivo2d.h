@@ -222,6 +222,21 @@ int fivo_project_shear(fivo_ctx* ctx, double* max_rel, double* mean_rel);
 /* run_sim_2d! (main2D.jl:371-561), device resident */
 int fivo_run(fivo_ctx* ctx, const fivo_run_opts* o, fivo_run_result* r);
 
+/* ---- readout: a background for a transport code ------------------------- */
+/* One time slice of export_background2d (tools/export_background2d.jl:79-94),
+ * from the primitives of the last primitive pass (call fivo_update_primitives at
+ * the slice's tau first, as the reference does). For every interior cell
+ * (i, j) = (ix - nghost, iy - nghost):
+ *     T = exp(yT),  v^x = u^x / u^tau,  v^y = u^y / u^tau,  n,  u^tau = sqrt(1 + u.u)
+ * (the VELOCITY a transport code reads, not the four-velocity), stored at element
+ * (i, j, k) of column-major (Nx, Ny, nt) tables: offset i + Nx (j + Ny k), Julia's
+ * T_grid[i, j, k]. device != 0: the pointers are device memory and the tables stay
+ * in HBM across slices; 0: host memory. n may be NULL. maxima (nullable) receives
+ * max |u| and max |v| over the slice (NaN if any cell's is NaN), for the
+ * exporter's |v| < 1 check. */
+int fivo_background_slice(fivo_ctx* ctx, int32_t k, int32_t nt, double* T, double* vx, double* vy,
+                          double* n, int32_t device, double* maxima);
+
 /* ---- timing hooks for benchmarks --------------------------------------- */
 /* Total device time (ms) of the kernels by category since fivo_enable_timing,
  * measured with HIP events on the context stream: 0 = primitive pass (warm

@@ -173,6 +173,31 @@ int orc_update_primitives(Ctx* c, double tau, int bc, fivo_counts* cnt) {
     if (cnt) { cnt->nprimfail = c->w.primfail; cnt->nvacuum = c->w.vacuum; cnt->amax = c->w.amax; }
     return FIVO_OK;
 }
+// export_background2d's slice loop (tools/export_background2d.jl:79-94): T = exp(yT),
+// v^i = u^i / u^tau, n of every interior cell at (i, j, k) of column-major (Nx, Ny, nt)
+// tables; max |u| and max |v| with Julia's max (a NaN propagates). Host tables only.
+int orc_background_slice(Ctx* c, int32_t k, int32_t nt, double* T, double* vx, double* vy, double* n,
+                         int32_t /*device*/, double* maxima) {
+    if (!T || !vx || !vy || nt < 1 || k < 0 || k >= nt) return FIVO_ERR_ARG;
+    const Grid& g = c->g;
+    const Work& w = c->w;
+    const auto jmax = [](double a, double b) { return (std::isnan(a) || std::isnan(b)) ? NaNv : (a > b ? a : b); };
+    double maxu = 0.0, maxv = 0.0;
+    for (int i = 0; i < g.Nx; ++i)
+        for (int j = 0; j < g.Ny; ++j) {
+            const int l = g.lin(i + g.ng, j + g.ng);
+            const double ux = w.ux[l], uy = w.uy[l], ut = std::sqrt(1 + ux * ux + uy * uy);
+            const size_t o = (size_t)i + (size_t)g.Nx * ((size_t)j + (size_t)g.Ny * k);
+            T[o] = std::exp(w.yT[l]);
+            vx[o] = ux / ut;
+            vy[o] = uy / ut;
+            if (n) n[o] = w.n[l];
+            maxu = jmax(maxu, std::hypot(ux, uy));
+            maxv = jmax(maxv, std::hypot(vx[o], vy[o]));
+        }
+    if (maxima) { maxima[0] = maxu; maxima[1] = maxv; }
+    return FIVO_OK;
+}
 int orc_sanitize(Ctx* c, double tau, int bc) { sanitize_stage(*c, c->U, tau, bc); return FIVO_OK; }
 int orc_state_ok(Ctx* c, double tau, int* ok, int64_t* nbad) {
     *ok = state_ok(*c, c->U, tau, nbad) ? 1 : 0;
