@@ -10,6 +10,10 @@
                       initialize_from_grid_csv! (main2D.jl:205-301): optional
                       Gaussian blur of the file grid, bilinear interpolation,
                       vacuum outside the file, radial taper
+  radial_profile      the interpolants of a radial `r,T0,alpha0` profile file,
+                      load_initial_interpolants (src/io.jl:51-258)
+  elliptic_profile    that profile on an area-preserving elliptical radius, the
+                      non-axisymmetric IC of gate G6 (test/test_elliptic2d.jl:48-60)
 
 Every function returns arrays over all Ntot cells (ghosts included), flattened in
 the library's flat order i = ix*Nytot + iy, ready for Solver.init_from_prims.
@@ -230,3 +234,93 @@ def write_grid_csv(path, xs, ys, T, alpha):
     X, Y = np.meshgrid(xs, ys, indexing="ij")
     rows = np.column_stack([X.ravel(), Y.ravel(), np.asarray(T).ravel(), np.asarray(alpha).ravel()])
     np.savetxt(path, rows, delimiter=",", header="x,y,T0,alpha0", comments="", fmt="%.17g")
+
+
+# ---- radial profile file -----------------------------------------------------------
+def radial_profile(csv, fugacity_kind="alpha", taper_width=1.0, interp_kind="linear", dup_r_tol=0.0):
+    """load_initial_interpolants (src/io.jl:51-258) for a CSV with columns r, T0 and
+    alpha0 (or lambda0), optionally nu_r0 / nu_r (physical, converted to the stored
+    sign, which is the same here: DISS_SIGN = +1, src/constants.jl:69) or nur0 / nur
+    (stored). Rows are sorted by r (stable), repeated r (within dup_r_tol) keep their
+    first row, and every column is interpolated linearly in r. Returns (itpT, itpF,
+    itpNur, rmax_data), vectorised functions of r >= 0:
+      r <= r_min: the value at r_min (T floored at T_MIN);
+      r_min < r <= rmax_data: the interpolant, blended over the last taper_width fm by
+        w = 1 - smoothstep01_5((r - r_t0)/taper_width), r_t0 = max(rmax_data - taper_width,
+        r_min), towards T_MIN, the vacuum fugacity (alpha 0, lambda 1) and nu 0;
+      r > rmax_data: T_MIN, the vacuum fugacity, 0.
+    Only interp_kind = "linear" is built (the reference's optional cubic resampling is
+    not)."""
+    if interp_kind != "linear":
+        raise ValueError(f"radial_profile: interp_kind {interp_kind!r} is not supported (only 'linear')")
+    if fugacity_kind not in ("alpha", "lambda"):
+        raise ValueError("fugacity_kind must be 'alpha' or 'lambda'")
+    with open(csv) as fh:
+        header = [h.strip() for h in fh.readline().strip().split(",")]
+    raw = np.loadtxt(csv, delimiter=",", skiprows=1, ndmin=2)
+    col = {h: raw[:, k] for k, h in enumerate(header)}
+    fname = "alpha0" if fugacity_kind == "alpha" else "lambda0"
+    for need in ("r", "T0", fname):
+        if need not in col:
+            raise ValueError(f"radial_profile: column {need!r} missing from {csv}")
+    r, T0, f0 = col["r"], col["T0"], col[fname]
+    nur = None
+    for k in ("nu_r0", "nu_r", "nur0", "nur"):
+        if k in col:
+            nur = col[k].copy()
+            break
+    if nur is None:
+        nur = np.zeros_like(r)
+    if len(r) < 2:
+        raise ValueError(f"Initial profile CSV must contain at least 2 rows; got {len(r)}.")
+    if not (np.all(np.isfinite(r)) and np.all(np.isfinite(T0)) and np.all(np.isfinite(f0)) and np.all(np.isfinite(nur))):
+        raise ValueError("Initial profile CSV contains NaN/Inf values.")
+    p = np.argsort(r, kind="stable")
+    r, T0, f0, nur = r[p], T0[p], f0[p], nur[p]
+    keep = np.ones(len(r), dtype=bool)
+    tol = max(dup_r_tol, 0.0)
+    keep[1:] = np.abs(np.diff(r)) > tol
+    r, T0, f0, nur = r[keep], T0[keep], f0[keep], nur[keep]
+    if len(r) < 2:
+        raise ValueError("After removing duplicate r entries, fewer than 2 points remain.")
+    if fugacity_kind == "lambda":
+        f0 = np.maximum(f0, 1e-300)
+    rmin_data, rmax_data = float(r[0]), float(r[-1])
+    T_floor = T_MIN
+    f_vac = 0.0 if fugacity_kind == "alpha" else 1.0
+    tw = max(taper_width, 0.0)
+    r_t0 = max(rmax_data - tw, rmin_data)
+    inv_tw = 1.0 / tw if tw > 0 else 0.0
+
+    def lin(v, x):
+        j = np.clip(np.searchsorted(r, x, side="right") - 1, 0, len(r) - 2)
+        t = (x - r[j]) / (r[j + 1] - r[j])
+        return (1.0 - t) * v[j] + t * v[j + 1]
+
+    def make(v, floor, vac):
+        v0 = float(v[0])
+
+        def f(rr):
+            x = np.asarray(rr, dtype=np.float64)
+            inner = lin(v, np.clip(x, rmin_data, rmax_data))
+            if tw > 0:
+                w = 1.0 - _smoothstep01_5((x - r_t0) * inv_tw)
+                tap = w * inner + (1.0 - w) * (vac if floor is None else floor)
+                inner = np.where(x >= r_t0, tap, inner)
+            out = np.where(x <= rmin_data, v0, np.where(x <= rmax_data, inner, vac if floor is None else floor))
+            return np.maximum(out, floor) if floor is not None else out
+        return f
+
+    return make(T0, T_floor, None), make(f0, None, f_vac), make(nur, None, 0.0), rmax_data
+
+
+def elliptic_profile(g, csv, eps2=0.25, taper_width=1.0):
+    """Gate G6's IC (test/test_elliptic2d.jl:48-60): the radial profile of `csv`
+    evaluated at r = hypot(x/a, y/b), a = sqrt(1 - eps2), b = sqrt(1 + eps2) (area
+    preserving), at rest."""
+    itpT, itpF, _, _ = radial_profile(csv, taper_width=taper_width)
+    a, b = math.sqrt(1 - eps2), math.sqrt(1 + eps2)
+    X, Y = _mesh(g)
+    r = np.hypot(X / a, Y / b)
+    n = g.Ntot
+    return dict(T=itpT(r).ravel(), alpha=itpF(r).ravel(), ux=np.zeros(n), uy=np.zeros(n))
