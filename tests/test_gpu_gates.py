@@ -30,6 +30,32 @@ def test_gubser_gate_g1_on_gpu(oracle, hip):
         assert h[2] < 1e-12
 
 
+def test_gubser_gate_g1_three_resolutions_on_gpu(hip):
+    """G1 at N = 100 / 200 / 400 (test/test_gubser2d.jl:203-245): the published
+    L2(T) 3.29e-3 -> 8.81e-4 -> 2.26e-4, L2(u) 7.73e-3 -> 1.92e-3 -> 4.57e-4, orders
+    T 1.90, 1.96 and u 2.01, 2.07, the entropy outflow -0.116 and its error against
+    the analytic disc entropy 1.28e-3 -> 9.56e-5 (README2D.md:275); x<->y < 1e-12,
+    ring spread converging and < 5e-3."""
+    rs = [G.run_gubser_full(hip, N) for N in (100, 200, 400)]
+    for N, r in zip((100, 200, 400), rs):
+        print(f"N={N}: L2(T) {r['L2T']:.3e} L2(u) {r['L2U']:.3e} x<->y {r['asym']:.1e} ring {r['ring']:.3e} "
+              f"S outflow {r['dS']:+.4f} S err {r['eS']:+.3e}", flush=True)
+        assert r["ncmp"] > 100 and r["asym"] < 1e-12
+    oT = [np.log2(rs[k]["L2T"] / rs[k + 1]["L2T"]) for k in range(2)]
+    oU = [np.log2(rs[k]["L2U"] / rs[k + 1]["L2U"]) for k in range(2)]
+    print(f"orders T {oT[0]:.2f}, {oT[1]:.2f}  u {oU[0]:.2f}, {oU[1]:.2f}", flush=True)
+    # each published figure to within 0.6 of its last printed digit (the reference
+    # rounds, e.g. L2(u) 1.915e-3 is printed 1.92e-3)
+    near = lambda x, p, unit: abs(x - p) <= 0.6 * unit
+    for r, pT, pU in zip(rs, (3.29e-3, 8.81e-4, 2.26e-4), (7.73e-3, 1.92e-3, 4.57e-4)):
+        assert near(r["L2T"], pT, 10 ** (np.floor(np.log10(pT)) - 2))
+        assert near(r["L2U"], pU, 10 ** (np.floor(np.log10(pU)) - 2))
+    assert all(near(o, p, 0.01) for o, p in zip(oT + oU, (1.90, 1.96, 2.01, 2.07)))
+    assert near(rs[2]["dS"], -0.116, 0.001)
+    assert near(abs(rs[0]["eS"]), 1.28e-3, 1e-5) and near(abs(rs[2]["eS"]), 9.56e-5, 1e-7)
+    assert rs[2]["ring"] < rs[0]["ring"] and rs[2]["ring"] < 5e-3
+
+
 def test_bjorken_gate_g0_on_gpu(oracle, hip):
     ou = OracleUnit(oracle)
     eos = F.LatticeHRGEOS()
@@ -78,6 +104,9 @@ def test_charge_gates_g3_on_gpu(oracle, hip):
     assert per["r"]["nprimfail"] == 0 and dper < 1e-12
     on = D.run_bump(hip)
     ref = D.run_bump(oracle)
+    nsame, ntot = D.upgradient_cells(on)
+    print(f"cells with nu^x up the gradient: {nsame} / {ntot} (published 0 of 2104)", flush=True)
+    assert nsame == 0 and ntot == 2104
     g, m = on["g"], on["m"]
     sx, sy = g.interior()
     inside = np.zeros((g.Nxtot, g.Nytot), bool)

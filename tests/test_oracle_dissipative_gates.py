@@ -305,16 +305,13 @@ def test_charge_conserved_on_closed_domain_g3c(oracle):
     assert dout > 1e3 * max(dper, 1e-16)      # the outflow run does lose charge: the box is open
 
 
-def test_diffusion_runs_down_the_gradient_g3d(oracle):
-    on = run_bump(oracle)
-    off = run_bump(oracle, enable_diff=False)
-    g, L = on["g"], on["m"].layout
-    ic0 = g.lin(g.nghost + g.Nx // 2 - 1, g.nghost + g.Ny // 2 - 1)     # lin(g, ng+Nx/2, ng+Ny/2), 1-based
-    peak_on, peak_off = on["U"][ic0, 0], off["U"][ic0, 0]
-    print(f"central tau J^tau: diffusion off {peak_off:.6e} on {peak_on:.6e}")
-    assert peak_on < peak_off
-    alpha = on["s"].get_work(("alpha",))["alpha"]
-    nux = on["U"][:, L.idx("nux")]
+def upgradient_cells(run):
+    """G3d's count (test/test_charge2d.jl:171-183): interior cells off the edge ring
+    with a resolvable central difference of alpha in x and a resolvable nu^x, and of
+    those the ones whose nu^x points up the gradient."""
+    g, L = run["g"], run["m"].layout
+    alpha = run["s"].get_work(("alpha",))["alpha"]
+    nux = run["U"][:, L.idx("nux")]
     ng, nyt = g.nghost, g.Nytot
     nsame = ntot = 0
     for ix in range(ng + 1, ng + g.Nx - 1):
@@ -325,8 +322,21 @@ def test_diffusion_runs_down_the_gradient_g3d(oracle):
                 continue
             ntot += 1
             nsame += nux[i] * dxa > 0
+    return nsame, ntot
+
+
+def test_diffusion_runs_down_the_gradient_g3d(oracle):
+    on = run_bump(oracle)
+    off = run_bump(oracle, enable_diff=False)
+    g, L = on["g"], on["m"].layout
+    ic0 = g.lin(g.nghost + g.Nx // 2 - 1, g.nghost + g.Ny // 2 - 1)     # lin(g, ng+Nx/2, ng+Ny/2), 1-based
+    peak_on, peak_off = on["U"][ic0, 0], off["U"][ic0, 0]
+    print(f"central tau J^tau: diffusion off {peak_off:.6e} on {peak_on:.6e}")
+    assert peak_on < peak_off
+    nsame, ntot = upgradient_cells(on)
     print(f"cells with nu^x up the gradient: {nsame} / {ntot}")
     assert ntot > 100 and nsame == 0
+    assert ntot == 2104                     # the published count (README2D.md:281: 0 of 2104)
 
 
 def test_radial_bump_stays_mirror_symmetric_g3e(oracle):
