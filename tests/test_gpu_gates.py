@@ -67,6 +67,10 @@ def test_bjorken_gate_g0_on_gpu(oracle, hip):
     assert abs(r["E"] - e) / e < 5e-4 and abs(r["T"] - T) / T < 5e-4 and abs(r["n"] - n) / n < 5e-4
     o = G.run_bjorken(oracle, 0.005)
     assert abs(r["T"] - o["T"]) <= 1e-9 * o["T"] and abs(r["E"] - o["E"]) <= 1e-9 * o["E"]
+    # the published temporal orders (README2D.md:273)
+    p2, p3 = G.bjorken_order(hip, ou, eos, "ssprk2"), G.bjorken_order(hip, ou, eos, "ssprk3")
+    print(f"orders SSPRK2 {p2:.3f} SSPRK3 {p3:.3f}", flush=True)
+    assert f"{p2:.2f}" == "2.00" and f"{p3:.2f}" == "2.99"
 
 
 @pytest.mark.parametrize("zs", [0.05, 0.15, 0.30])
@@ -95,6 +99,11 @@ def test_viscous_bjorken_gate_g2c_on_gpu(oracle, hip):
     assert abs(h["e"] - o["e"]) <= 1e-9 * o["e"] and abs(h["pixx"] - o["pixx"]) <= 1e-9 * abs(o["pixx"])
     ideal = D.run_visc_bjorken(hip, eta_over_s=0.0, referee=False)
     assert h["pieta"] < 0.0 and h["e"] > ideal["e"]
+    # G2e: the operator split is first order, the published 1.05 (README2D.md:277)
+    errs = [D.run_visc_bjorken(hip, CFLtau=c)["eerr"] for c in (0.016, 0.008, 0.004)]
+    p = np.log2(errs[1] / errs[2])
+    print(f"G2e order {p:.3f}", flush=True)
+    assert f"{p:.2f}" == "1.05"
 
 
 def test_charge_gates_g3_on_gpu(oracle, hip):

@@ -244,6 +244,7 @@ def test_operator_split_converges_g2e(oracle):
     p = math.log2(errs[1] / errs[2])
     print("errors", errs, "order", p)
     assert p > 0.7
+    assert f"{p:.2f}" == "1.05"         # the published order (README2D.md:277)
 
 
 # ---- symmetry and conservation with every sector on ---------------------------
