@@ -227,6 +227,21 @@ int fivo_batch_autotune(fivo_batch* bp, double tau, int32_t* tx, int32_t* ty, fl
     Batch& b = *reinterpret_cast<Batch*>(bp);
     int rc;
     if ((rc = sync_events(b)) != FIVO_OK) return rc;
+    // the timing needs real primitives, but a primitive pass moves the Newton warm
+    // starts: keep them in each event's snapshot buffer (nv >= 4 rows) and put them
+    // back afterwards, so tuning leaves the solver state as it found it
+    auto x0_copy = [&](bool save) -> int {
+        for (Ctx* c : b.evs) {
+            double* x0[4] = {c->w.x0_yT, c->w.x0_phi, c->w.x0_ux, c->w.x0_uy};
+            for (int k = 0; k < 4; ++k) {
+                double* keep = c->snap + (size_t)k * c->p.pitch;
+                BCHK(&b, hipMemcpyAsync(save ? keep : x0[k], save ? x0[k] : keep, sizeof(double) * c->p.Ntot,
+                                        hipMemcpyDeviceToDevice, b.stream));
+            }
+        }
+        return FIVO_OK;
+    };
+    if ((rc = x0_copy(true)) != FIVO_OK) return rc;
     if ((rc = prim_at(b, tau)) != FIVO_OK) return rc;
     hipEvent_t t0, t1;
     BCHK(&b, hipEventCreate(&t0));
@@ -252,6 +267,7 @@ int fivo_batch_autotune(fivo_batch* bp, double tau, int32_t* tx, int32_t* ty, fl
     (void)hipEventDestroy(t1);
     b.tile = best;
     if ((rc = build_maps(b)) != FIVO_OK) return rc;
+    if ((rc = x0_copy(false)) != FIVO_OK) return rc;
     int X, Y;
     batch_tile_shape(best, &X, &Y);
     if (tx) *tx = X;

@@ -135,6 +135,72 @@ def export_background2d(ic, out=None, N=200, tau0=0.4, tauf=8.0, nt=39, xmax=16.
     return res
 
 
+def export_backgrounds2d(ics, N=200, tau0=0.4, tauf=8.0, nt=39, xmax=16.0, DsT=0.1163, eta_over_s=0.10,
+                         zeta_over_s=0.10, lib=None, CFL=0.2, CFLtau=0.02, learned=None, autotune=True,
+                         verbose=False):
+    """export_background2d for many events at once: every event's chunk between two
+    slice times is one ensemble run (one launch per phase over all events, the step
+    replayed as a hipGraph, each event with its own dt), then each event's slice is
+    written into its own tables in HBM. `ics` is a list of what export_background2d
+    takes as `ic`; `N` one size or one per event. The ensemble integrates with
+    SSPRK3, so each event's tables equal export_background2d(..., integrator="ssprk3")
+    bit for bit. `learned`: an Mlp to seed cold recoveries (results then agree with
+    the plain path to the learned seed's tolerance, not bit for bit). Returns one
+    result dict per event, with the exporter's keys."""
+    from .batch import Ensemble
+    lib = lib if lib is not None else _abi.load_hip()
+    if not _is_device_lib(lib):
+        raise ValueError("export_backgrounds2d needs the HIP library (ensembles run on the device)")
+    Ns = [int(N)] * len(ics) if np.ndim(N) == 0 else [int(n) for n in N]
+    grids = [make_grid2d(n, n, xmax=xmax, ymax=xmax) for n in Ns]
+    m = background_model(DsT, eta_over_s, zeta_over_s)
+    ens = Ensemble(grids, m, lib=lib)
+    t_start = time.perf_counter()
+    try:
+        names = [_init(ev, g, ic, tau0) for ev, g, ic in zip(ens.events, grids, ics)]
+        if learned is not None:
+            ens.set_learned_seed(learned)
+        if autotune:
+            ens.autotune(tau0)
+        ts = np.linspace(tau0, tauf, nt)
+        tabs = [_Tables(g, nt, True) for g in grids]
+        taus = [tau0] * len(ics)
+        steps = [0] * len(ics)
+        maxu = [0.0] * len(ics)
+        maxv = [0.0] * len(ics)
+        jmax = lambda a, b: math.nan if (math.isnan(a) or math.isnan(b)) else max(a, b)
+        for k, tk in enumerate(ts):
+            if tk > min(taus) + 1e-12:
+                rs = ens.run_sim_2d(tau0=taus, taufinal=float(tk), CFL=CFL, CFLtau=CFLtau, reset_history=False)
+                for e, r in enumerate(rs):
+                    if not r["ok"]:
+                        raise FivoError(f"export_backgrounds2d: event {e}'s solve failed at tau = {r['tau']}")
+                    taus[e] = r["tau"]
+                    steps[e] += r["nsteps"]
+            for e, ev in enumerate(ens.events):
+                ev.update_primitives_2d(taus[e])
+                mu, mv = tabs[e].slice(ev, k)
+                maxu[e], maxv[e] = jmax(maxu[e], mu), jmax(maxv[e], mv)
+        hosts = [t.host() for t in tabs]
+    finally:
+        ens.close()
+    wall = time.perf_counter() - t_start
+    out = []
+    for e, (g, (T, vx, vy, n)) in enumerate(zip(grids, hosts)):
+        if not maxv[e] < 1.0:
+            raise FivoError(f"export_backgrounds2d: event {e}: |v| = {maxv[e]} >= 1")
+        ng = g.nghost
+        res = dict(label=LABEL, xgrid=np.array(g.xC[ng:ng + g.Nx]), ygrid=np.array(g.yC[ng:ng + g.Ny]),
+                   tgrid=ts, T_grid=T, vx_grid=vx, vy_grid=vy, n_grid=n, ic=str(names[e]), N=Ns[e], xmax=xmax,
+                   tau0=tau0, tauf=tauf, DsT=DsT, eta_over_s=eta_over_s, zeta_over_s=zeta_over_s,
+                   velocity_convention=VELOCITY_CONVENTION, note=NOTE)
+        res["_run"] = {"steps": steps[e], "wall_s": wall, "max_u": maxu[e], "max_v": maxv[e]}
+        out.append(res)
+    if verbose:
+        print(f"FiVo 2-D backgrounds: {len(ics)} events, {sum(steps)} event-steps in {wall:.2f} s", flush=True)
+    return out
+
+
 def load_background(path):
     """Read a file written by export_background2d(out=...) (no pickle)."""
     with np.load(path, allow_pickle=False) as z:

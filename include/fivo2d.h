@@ -301,7 +301,9 @@ const char* fivo_batch_last_error(const fivo_batch* b);  /* b may be NULL */
 fivo_ctx* fivo_batch_event(fivo_batch* b, int32_t ev);
 /* time the fused face kernel for every candidate tile shape on the current
  * states at tau (after a primitive pass there) and keep the fastest; ms gets
- * one time per candidate (FIVO_BATCH_NTILES), tx/ty the chosen tile */
+ * one time per candidate (FIVO_BATCH_NTILES), tx/ty the chosen tile. The Newton
+ * warm starts that pass moves are restored, so a later run is the same with or
+ * without tuning. */
 #define FIVO_BATCH_NTILES 4
 int fivo_batch_autotune(fivo_batch* b, double tau, int32_t* tx, int32_t* ty, float* ms);
 /* use candidate tile `tile` (0..FIVO_BATCH_NTILES-1) without timing */

@@ -161,3 +161,23 @@ def test_ensemble_continues_from_per_event_times(hip):
         assert a1["tau"] == r1[k]["tau"] and a2["tau"] == r2[k]["tau"] and a2["nsteps"] == r2[k]["nsteps"] == 3
         assert np.array_equal(s.get_state(), ens.events[k].get_state()), k
     ens.close()
+
+
+def test_autotune_leaves_the_solver_state_unchanged(hip):
+    """Tuning runs a primitive pass to time the face tiles on real states; the warm
+    starts that pass moves are restored, so a run after tuning is the run without."""
+    m = full_model()
+    evs = events()
+    ens = F.Ensemble([g for g, _ in evs], m, lib=hip)
+    for s, (g, prims) in zip(ens.events, evs):
+        s.init_from_prims(RUN["tau0"], **prims)
+    keys = ("x0_yT", "x0_phi", "x0_ux", "x0_uy")
+    before = [s.get_work(keys) for s in ens.events]
+    U0 = [s.get_state() for s in ens.events]
+    ens.autotune(RUN["tau0"])
+    for k, s in enumerate(ens.events):
+        after = s.get_work(keys)
+        for f in keys:
+            assert np.array_equal(before[k][f], after[f], equal_nan=True), (k, f)
+        assert np.array_equal(U0[k], s.get_state()), k
+    ens.close()
