@@ -117,39 +117,66 @@ def full_model():
                             kappa_coeff=0.1163, pi_clip_factor=1.0, consistent_m2=True)
 
 
-def train_warm_start(lib):
-    """The learned warm start: pairs the solver produces on 6 training events
-    (their own seeds, none of the benchmark events), then Adam on the GPU."""
+def _training_sources():
+    """Training events of the three viscous + charm configurations, on seeds none of
+    the benchmark events use: (label, N, L, spots, sigma, tau0, integrator, m2, seed)."""
+    src = []
+    for k in range(6):      # configuration 2: 256^2 on [-12,12], 150 spots, sigma 0.5, SSPRK2, no m2
+        src.append(("c2", 256, 12.0, 150, 0.5, 0.6, "ssprk2", False, 7100 + k))
+    for k in range(3):      # configuration 3: 512^2 on [-15,15], 300 spots, sigma 0.4, SSPRK3, full sectors
+        src.append(("c3", 512, 15.0, 300, 0.4, 0.4, "ssprk3", True, 7200 + k))
+    sizes = (128, 192, 256, 320, 384, 256)
+    for k in range(6):      # configuration 4: uneven sizes, 50-300 spots
+        src.append(("c4", sizes[k], 15.0, 50 + 50 * k, 0.4, 0.4, "ssprk3", True, 7300 + k))
+    return src
+
+
+def train_warm_start(lib, target_pairs=10_000_000, passes=24):
+    """The learned warm start: (conserved, primitive) pairs the solver produces on
+    training events of configurations 2-4 (their own seeds, none of the benchmark
+    events), collected at the start and every two steps (up to `passes` times per
+    event) until `target_pairs`, then Adam on the GPU with the project's kernels."""
     import fivo2d as F
-    m = full_model()
-    net = F.Mlp(capacity=4_000_000, seed=3, lib=lib)
+    net = F.Mlp(capacity=target_pairs, seed=3, lib=lib)
     t0 = time.perf_counter()
-    for k in range(6):
-        g = F.make_grid2d(192, 192, xmax=ENS_COMMON["L"], ymax=ENS_COMMON["L"])
+    per_cfg = {}
+    for label, N, L, nsp, sig, tau0, integ, m2, seed in _training_sources():
+        if net.npairs >= target_pairs:
+            break
+        g = F.make_grid2d(N, N, xmax=L, ymax=L)
+        m = F.build_model_2d(eos=F.LatticeHRGEOS(), enable_shear=True, eta_over_s=0.08, tauShear_coeff=0.2,
+                             enable_bulk=True, zeta_over_s=0.1, tauPi_coeff=15.0, enable_diff=True,
+                             kappa_coeff=0.1163, pi_clip_factor=1.0, consistent_m2=m2)
         s = F.Solver(g, m, lib=lib)
-        s.init_hotspots(ENS_COMMON["tau0"], seed=7000 + k, nspots=50 + 40 * k, sigma=ENS_COMMON["sigma"], Tpeak=0.45)
-        tau = ENS_COMMON["tau0"]
+        s.init_hotspots(tau0, seed=seed, nspots=nsp, sigma=sig, Tpeak=0.45)
+        tau = tau0
         s.update_primitives_2d(tau)
+        n0 = net.npairs
         net.collect(s, tau)
-        for j in range(12):
-            r = s.run_sim_2d(tau0=tau, taufinal=1e9, CFL=ENS_COMMON["cfl"], CFLtau=ENS_COMMON["cfltau"],
-                             integrator="ssprk3", max_steps=2, reset_history=(j == 0))
+        for j in range(passes - 1):
+            if net.npairs >= target_pairs:
+                break
+            r = s.run_sim_2d(tau0=tau, taufinal=1e9, CFL=0.2, CFLtau=0.05, integrator=integ, max_steps=2,
+                             reset_history=(j == 0))
             tau = r["tau"]
             net.collect(s, tau)
+        per_cfg[label] = per_cfg.get(label, 0) + net.npairs - n0
         s.close()
     t_collect = time.perf_counter() - t0
     net.fit_normalization()
     t0 = time.perf_counter()
     losses = []
-    for epochs, lr in ((15, 3e-3), (10, 1e-3), (5, 3e-4)):
+    for epochs, lr in ((8, 3e-3), (6, 1e-3), (3, 3e-4)):
         losses += list(net.train(epochs=epochs, batch=4096, lr=lr))
     t_train = time.perf_counter() - t0
-    log(f"warm start: {net.npairs} pairs in {t_collect:.1f} s, trained in {t_train:.1f} s")
+    log(f"warm start: {net.npairs} pairs in {t_collect:.1f} s ({per_cfg}), trained in {t_train:.1f} s")
     mse = net.eval()
-    info = {"pairs": net.npairs, "collect_s": round(t_collect, 3), "train_s": round(t_train, 3),
-            "epochs": len(losses), "loss_first_last": [float(losses[0]), float(losses[-1])],
+    info = {"pairs": net.npairs, "pairs_by_configuration": per_cfg, "collect_s": round(t_collect, 3),
+            "train_s": round(t_train, 3), "epochs": len(losses),
+            "loss_first_last": [float(losses[0]), float(losses[-1])],
             "mse_normalised": {k: float(v) for k, v in zip(("yT", "phi", "asinh_us", "u_perp", "log_e", "log_P"), mse)},
-            "training_events": "6 events 192^2, 50-250 spots, seeds 7000-7005, 13 primitive passes each"}
+            "training_events": "configuration 2 (6 x 256^2, SSPRK2), 3 (3 x 512^2) and 4 (6 events 128-384^2), "
+                               "seeds 7100-7305, a pair set every two steps"}
     return net, info
 
 
