@@ -111,6 +111,12 @@ void orc_debug_iters(Ctx* c, int enable, int64_t* hist256) {
 void orc_debug_hist2(Ctx* c, int64_t* h) {
     for (int k = 0; k < 2 * 8 * 256; ++k) h[k] = k < (int)c->w.dbg_hist2.size() ? c->w.dbg_hist2[k] : 0;
 }
+// expand_bad_2d! on a host mask of Ntot bytes, in place (unit checks of the MOOD dilation)
+void orc_debug_expand(Ctx* c, uint8_t* bad, int radius) {
+    std::copy(bad, bad + c->g.Ntot, c->w.bad.begin());
+    expand_bad(*c, radius);
+    std::copy(c->w.bad.begin(), c->w.bad.end(), bad);
+}
 // per-cell iterations and class of the last primitive pass (recording enabled)
 void orc_debug_cells(Ctx* c, int32_t* iters, int32_t* cls) {
     const size_t n = c->w.dbg_iters.size();

@@ -519,6 +519,24 @@ bool state_ok(Ctx& c, const std::vector<double>& U, double tau, int64_t* nbad) {
     return cnt == 0;
 }
 
+// expand_bad_2d! (src2d/floors2d.jl:369-383): `radius` rounds of growing the mask
+// by the four neighbours of every marked cell, interior cells only
+void expand_bad(Ctx& c, int radius) {
+    const Grid& g = c.g;
+    Work& w = c.w;
+    const int ng = g.ng;
+    for (int r = 0; r < radius; ++r) {
+        w.bad_tmp = w.bad;
+        for (int ix = ng; ix < ng + g.Nx; ++ix)
+            for (int iy = ng; iy < ng + g.Ny; ++iy) {
+                int i = g.lin(ix, iy);
+                if (w.bad_tmp[i]) continue;
+                if (w.bad[i - 1] || w.bad[i + 1] || w.bad[i - g.Nytot] || w.bad[i + g.Nytot]) w.bad_tmp[i] = 1;
+            }
+        w.bad = w.bad_tmp;
+    }
+}
+
 // mark_bad_2d! + expand_bad_2d! (src2d/floors2d.jl:350-383)
 static void mark_and_expand(Ctx& c, const std::vector<double>& U, double tau, int radius) {
     const Grid& g = c.g;
@@ -531,16 +549,7 @@ static void mark_and_expand(Ctx& c, const std::vector<double>& U, double tau, in
             int i = g.lin(ix, iy);
             if (!cell_admissible(c, U, i, tau, true)) w.bad[i] = 1;
         }
-    for (int r = 0; r < radius; ++r) {
-        w.bad_tmp = w.bad;
-        for (int ix = ng; ix < ng + g.Nx; ++ix)
-            for (int iy = ng; iy < ng + g.Ny; ++iy) {
-                int i = g.lin(ix, iy);
-                if (w.bad_tmp[i]) continue;
-                if (w.bad[i - 1] || w.bad[i + 1] || w.bad[i - g.Nytot] || w.bad[i + g.Nytot]) w.bad_tmp[i] = 1;
-            }
-        w.bad = w.bad_tmp;
-    }
+    expand_bad(c, radius);
 }
 
 // compute_dt_2d (src2d/timestepper2d.jl:37-47)

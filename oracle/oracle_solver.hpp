@@ -97,6 +97,7 @@ void enforce_nu_charge_constraint(Ctx& c, std::vector<double>& U, double tau, do
 void sanitize_stage(Ctx& c, std::vector<double>& U, double tau, int bc);
 bool cell_admissible(Ctx& c, const std::vector<double>& U, int i, double tau, bool use_work);
 bool state_ok(Ctx& c, const std::vector<double>& U, double tau, int64_t* nbad);
+void expand_bad(Ctx& c, int radius);
 double compute_dt(Ctx& c, double tau, double cfl, double cfltau);
 bool step_ssprk2(Ctx& c, double tau, double dtau, int bc, int max_halve, int max_mood, double* used);
 bool step_ssprk3(Ctx& c, double tau, double dtau, int bc, int max_halve, double* used);
