@@ -126,3 +126,75 @@ def test_64_event_ensemble_bit_identical_to_one_by_one(hip):
         assert r["nsteps"] == res[k]["nsteps"] and r["tau"] == res[k]["tau"]
         assert np.array_equal(s.get_state(), Ub[k]), k
     y.close()
+
+
+@pytest.mark.parametrize("name,tauf", [("c1", 3.0), ("c2", 8.0)])
+def test_full_length_run_vs_oracle(oracle, hip, name, tauf):
+    """The benchmark configurations 1 (ideal Gubser 128^2, tau 1 -> 3, SSPRK3) and 2
+    (viscous + charm 256^2 hot-spot event, tau 0.6 -> 8, SSPRK2 with MOOD) run to
+    their final times on both paths: the same steps and counters, no recovery-flag
+    flips, final T and u within 1e-9."""
+    import ctypes as C
+    import sys
+    from conftest import REPO
+    if REPO not in sys.path:
+        sys.path.insert(0, REPO)
+    import bench
+    cfg = bench.CONFIGS[name]
+    oracle.dll.orc_set_threads.restype = C.c_int
+    oracle.dll.orc_set_threads(16)
+    out = []
+    for lib in (hip, oracle):
+        g, m, prims = bench.make_inputs(cfg)
+        s = F.Solver(g, m, lib=lib)
+        s.init_from_prims(cfg["tau0"], **prims)
+        r = s.run_sim_2d(tau0=cfg["tau0"], taufinal=tauf, CFL=cfg["cfl"], CFLtau=cfg["cfltau"],
+                         integrator=cfg["integrator"])
+        assert r["ok"]
+        out.append((r, s.fields_2d(), s.get_work(("ok",))["ok"]))
+        s.close()
+    (rh, fh, okh), (ro, fo, oko) = out
+    for k in ("nsteps", "nhalvings", "nmood", "nprimfail", "tau"):
+        assert rh[k] == ro[k], k
+    assert np.array_equal(okh, oko)
+    dT = np.max(np.abs(fh["T"] - fo["T"]) / fo["T"])
+    du = max(np.max(np.abs(fh[k] - fo[k])) for k in ("ux", "uy"))
+    print(f"{name} to tau {tauf}: {rh['nsteps']} steps, max dT/T {dT:.2e}, max du {du:.2e}", flush=True)
+    assert dT <= 1e-9 and du <= 1e-9
+
+
+def test_full_length_c3_conditioning(hip):
+    """Configuration 3 (512^2 full sectors, tau 0.4 -> 10, SSPRK3) on its lumpy event is
+    ill-conditioned at full length: the same run with the initial conserved rows
+    perturbed by 1e-13 (relative) departs by far more than 1e-9, with recovery-flag
+    flips. So final-time agreement to 1e-9 with the reference on this configuration
+    is out of reach of any implementation that is not bit-identical to it; the HIP
+    path's measured distance from the oracle at full length (tests/run_full_length.py:
+    2.9e-7, no flips, the same 802 steps) is far inside this spread."""
+    import sys
+    from conftest import REPO
+    if REPO not in sys.path:
+        sys.path.insert(0, REPO)
+    import bench
+    cfg = bench.CONFIGS["c3"]
+    Ts, oks = [], []
+    for eps in (0.0, 1e-13):
+        g, m, prims = bench.make_inputs(cfg)
+        s = F.Solver(g, m, lib=hip)
+        s.init_from_prims(cfg["tau0"], **prims)
+        if eps:
+            U = s.get_state()
+            U[:, :4] *= 1 + eps * np.random.default_rng(1).uniform(-1, 1, size=(U.shape[0], 4))
+            s.set_state(U)
+        r = s.run_sim_2d(tau0=cfg["tau0"], taufinal=10.0, CFL=cfg["cfl"], CFLtau=cfg["cfltau"],
+                         integrator=cfg["integrator"])
+        assert r["ok"]
+        w = s.get_work(("yT", "ok"))
+        Ts.append(np.exp(w["yT"]))
+        oks.append(w["ok"])
+        s.close()
+    both = (oks[0] & oks[1]).astype(bool)
+    d = np.max(np.abs(Ts[1] - Ts[0])[both] / Ts[0][both])
+    flips = int(np.sum(oks[0] != oks[1]))
+    print(f"c3 to tau 10, initial state perturbed by 1e-13: max dT/T {d:.2e}, recovery-flag flips {flips}", flush=True)
+    assert d > 1e-6
