@@ -481,7 +481,7 @@ def main():
                     help="also run a device copy of 512 MiB (the reference kernel of the traffic counters)")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
+        print(f"bench.py: this benchmark runs on one GPU; --gpus {a.gpus} is not supported "
               f"(an ensemble's events would shard across GPUs; only one GPU is measured).", file=sys.stderr)
         sys.exit(2)
     import torch

@@ -1,7 +1,7 @@
 """GPU parity: the HIP library against the CPU oracle on identical inputs.
 
-Bar (from the task's north star): per-RHS relative difference <= 1e-12, final
-time relative difference <= 1e-9 in T and u. "Relative" is per field: the max
+Bar: per-RHS relative difference <= 1e-12, final-time relative difference
+<= 1e-9 in T and u. "Relative" is per field: the max
 abs difference over cells divided by the field's max abs value.
 """
 import numpy as np

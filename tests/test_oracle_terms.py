@@ -77,7 +77,7 @@ def test_presets_and_ingredients_gt7():
 
 def _charge_run(lib, **kw):
     """The Gt6/Gt7 solve: 24^2 on [-6, 6], a T lump and an offset fugacity lump,
-    tau 0.4 -> 0.9 with the driver defaults (test/test_terms2d.jl:322-334)."""
+    tau 0.4 -> 0.9 with run_sim_2d!'s defaults (test/test_terms2d.jl:322-334)."""
     g = F.make_grid2d(24, 24, xmax=6.0, ymax=6.0)
     m = F.build_model_2d(eos=F.LatticeHRGEOS(), enable_diff=True, kappa_coeff=0.1163, **kw)
     X, Y = np.meshgrid(g.xC, g.yC, indexing="ij")
