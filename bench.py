@@ -582,6 +582,19 @@ def main():
             if e in legs:
                 out[f"ensemble_{e}"] = ensemble_bench(lib, e, a.ensemble_steps, 3, net)
         net.close()
+    # the stated targets, read off the legs above: batched+learned against the plain
+    # path on the ensembles (with their final-time differences), and the plain 512^2
+    # step's algorithmic HBM rate as a fraction of the peak
+    tg = {"plain_step_hbm_fraction": out["roofline"]["step_achieved_GBs"] / HBM_PEAK_GBS}
+    for e in ("c4", "c5"):
+        x = out.get(f"ensemble_{e}")
+        if x:
+            bl = x["batched_learned"]
+            tg[f"ensemble_{e}_batched_learned_over_plain"] = x["speedup_batched_learned_vs_plain"]
+            tg[f"ensemble_{e}_max_dT_over_T"] = bl["max_dT_over_T_vs_plain"]
+            tg[f"ensemble_{e}_events_with_dT_over_1e-9_or_flips"] = bl["events_with_cells_dT_over_1e-9_or_flips"]
+            tg[f"ensemble_{e}_max_dT_over_T_other_events"] = bl["max_dT_over_T_other_events"]
+    out["targets"] = tg
     if "background" in legs:
         out["background_export"] = background_bench(lib, s, g)
     if not a.no_cpu:
