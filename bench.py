@@ -227,11 +227,11 @@ def background_bench(lib, s, g):
     log(f"background export: {wall:.2f} s ({run['steps']} steps); slice kernel {avg_ms * 1e3:.1f} us, "
         f"{out['slice_kernel']['achieved_GBs']:.0f} GB/s")
     # eight events at the same defaults: one ensemble run per slice interval against
-    # eight one-event exports, both SSPRK3 (the ensemble's integrator), same tables
+    # eight one-event exports, both with the exporter's SSPRK2, same tables
     evs = [lambda g, k=k: _bg_event_seed(g, 100 + k) for k in range(8)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    one = [F.export_background2d(f, lib=lib, integrator="ssprk3", verbose=False) for f in evs]
+    one = [F.export_background2d(f, lib=lib, verbose=False) for f in evs]
     w1 = time.perf_counter() - t0
     t0 = time.perf_counter()
     many = F.export_backgrounds2d(evs, lib=lib)
@@ -239,7 +239,7 @@ def background_bench(lib, s, g):
     same = all(np.array_equal(a[k], b[k]) for a, b in zip(one, many)
                for k in ("T_grid", "vx_grid", "vy_grid", "n_grid"))
     es = sum(r["_run"]["steps"] for r in many)
-    out["ensemble_8_events"] = {"workload": "8 seeded hot-spot events at the exporter's defaults, SSPRK3",
+    out["ensemble_8_events"] = {"workload": "8 seeded hot-spot events at the exporter's defaults (SSPRK2)",
                                 "one_by_one_s": w1, "ensemble_s": w8, "speedup": w1 / w8, "event_steps": es,
                                 "cell_updates_per_s": es * 200 * 200 / w8, "tables_identical": bool(same)}
     log(f"background export, 8 events: one by one {w1:.2f} s, ensemble {w8:.2f} s (x{w1 / w8:.2f}), "

@@ -89,13 +89,25 @@ int event_diag(Batch& b, Ctx& c, int what, double* q, double* maxu, double* minp
 int record_step(Batch& b) {
     BCHK(&b, launch_batch_ctl(b, 1));
     BCHK(&b, launch_batch_history(b));
-    for (int s = 1; s <= 3; ++s) {
+    const int ns = b.run.integ == FIVO_SSPRK2 ? 2 : 3;
+    for (int s = 1; s <= ns; ++s) {
         if (s == 1) BCHK(&b, launch_batch_bc(b, PH_STAGE1, 0));
         BCHK(&b, launch_batch_prim(b, s));
         BCHK(&b, launch_batch_faces(b, s, b.tile));
         BCHK(&b, launch_batch_bc(b, s, s));
         BCHK(&b, launch_batch_pending(b, s));
         BCHK(&b, launch_batch_ctl(b, 10 + s));
+        // MOOD retries (SSPRK2): every launch of retry a runs only the events whose
+        // check asked for it; the mask is the stage's bad mask expanded a times
+        for (int a = 1; a <= b.run.max_mood; ++a) {
+            const int ph = s + PH_RETRY * a;
+            for (int r = 0; r < a; ++r) BCHK(&b, launch_batch_expand(b, ph, r));
+            BCHK(&b, launch_batch_prim(b, ph));
+            BCHK(&b, launch_batch_faces(b, ph, b.tile));
+            BCHK(&b, launch_batch_bc(b, ph, s));
+            BCHK(&b, launch_batch_pending(b, ph));
+            BCHK(&b, launch_batch_ctl(b, 100 * a + 10 + s));
+        }
     }
     if (b.run.dissip) {
         BCHK(&b, launch_batch_bc(b, PH_REFRESH, 0));
@@ -279,7 +291,9 @@ int fivo_batch_autotune(fivo_batch* bp, double tau, int32_t* tx, int32_t* ty, fl
 int fivo_batch_run(fivo_batch* bp, const fivo_run_opts* o, const double* tau0, fivo_run_result* res) {
     Batch& b = *reinterpret_cast<Batch*>(bp);
     if (!o || !res) return bfail(&b, FIVO_ERR_ARG, "batch_run: null argument");
-    if (o->integrator != FIVO_SSPRK3) return bfail(&b, FIVO_ERR_ARG, "batch_run: ensembles step with SSPRK3");
+    if (o->integrator != FIVO_SSPRK3 && o->integrator != FIVO_SSPRK2)
+        return bfail(&b, FIVO_ERR_ARG, "batch_run: integrator must be SSPRK2 or SSPRK3");
+    if (o->max_mood < 0 || o->max_mood > 4) return bfail(&b, FIVO_ERR_ARG, "batch_run: max_mood must be 0..4");
     if (o->on_dump) return bfail(&b, FIVO_ERR_ARG, "batch_run: no dump callback for ensembles");
     int rc;
     if ((rc = sync_events(b)) != FIVO_OK) return rc;
@@ -306,6 +320,9 @@ int fivo_batch_run(fivo_batch* bp, const fivo_run_opts* o, const double* tau0, f
     }
     if ((rc = sync_events(b)) != FIVO_OK) return rc;
     b.run.bc = o->bc; b.run.max_halve = o->max_halve; b.run.dissip = dissip ? 1 : 0;
+    // step_ssprk3_2d! halves only; step_ssprk2_2d! retries a stage first (MOOD)
+    b.run.integ = o->integrator;
+    b.run.max_mood = o->integrator == FIVO_SSPRK2 ? o->max_mood : 0;
     b.run.cfl = o->cfl; b.run.cfltau = o->cfltau;
     BCHK(&b, hipMemcpyAsync(b.d_ctl, b.h_ctl, sizeof(EvCtl) * b.nev, hipMemcpyHostToDevice, b.stream));
     // the initial RHS at tau0 (run_sim_2d!, main2D.jl:384-400): amax for the first dt
@@ -339,7 +356,7 @@ int fivo_batch_run(fivo_batch* bp, const fivo_run_opts* o, const double* tau0, f
         BCHK(&b, hipMemcpy(c.sc_host, c.sc, sizeof(DevScalars), hipMemcpyDeviceToHost));
         r.tau = k.tau; r.nsteps = k.nsteps; r.nprimfail = k.nprimfail; r.nvacuum = k.nvacuum;
         r.max_shear_res = k.max_shear; r.Q0 = Q0[e];
-        r.nrhs = k.nrhs; r.nhalvings = k.nhalvings; r.nmood = 0;
+        r.nrhs = k.nrhs; r.nhalvings = k.nhalvings; r.nmood = k.nmood;
         r.newton_iters = (int64_t)c.sc_host->newton_iters;
         r.nlearned = (int64_t)c.sc_host->nlearned;
         r.nlearned_fallback = (int64_t)c.sc_host->nfallback;

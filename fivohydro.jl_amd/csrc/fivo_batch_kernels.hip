@@ -44,16 +44,22 @@ __device__ __forceinline__ DevWork ev_work(CEv E) {
 __device__ __forceinline__ int block_event(const BatchRun& R, int kind) {
     return __builtin_amdgcn_readfirstlane(((CInt)R.map[kind])[blockIdx.x]);
 }
+// a phase is a stage and, for a MOOD retry, its attempt (phase = stage + PH_RETRY attempt)
+__device__ __forceinline__ int ph_stage(int phase) { return phase & (PH_RETRY - 1); }
+__device__ __forceinline__ int ph_attempt(int phase) { return phase / PH_RETRY; }
 __device__ __forceinline__ bool ev_active(CCtl c, int phase) {
-    if (phase == PH_REFRESH) return c->accepted != 0;
-    return c->state == EV_RUNNING && c->stage == (phase == PH_INIT ? 0 : phase - 1);
+    const int ps = ph_stage(phase);
+    if (ps == PH_REFRESH) return c->accepted != 0;
+    return c->state == EV_RUNNING && c->stage == (ps == PH_INIT ? 0 : ps - 1) && c->mood_attempt == ph_attempt(phase);
 }
 // stage input role (offset from the accepted buffer) and the time of its primitive pass
 __device__ __forceinline__ int phase_in_role(int phase) {
-    return phase == PH_STAGE2 ? 1 : (phase == PH_STAGE3 ? 2 : 0);
+    const int ps = ph_stage(phase);
+    return ps == PH_STAGE2 ? 1 : (ps == PH_STAGE3 ? 2 : 0);
 }
 __device__ __forceinline__ double phase_tau(CCtl c, int phase) {
-    return phase == PH_STAGE2 ? c->tau + c->D : (phase == PH_STAGE3 ? c->tau + 0.5 * c->D : c->tau);
+    const int ps = ph_stage(phase);
+    return ps == PH_STAGE2 ? c->tau + c->D : (ps == PH_STAGE3 ? c->tau + 0.5 * c->D : c->tau);
 }
 
 // ---- ghost fill of buffer `role` -------------------------------------------------------
@@ -156,7 +162,7 @@ __global__ __launch_bounds__((face_threads<TX, TY>())) void kb_faces(BatchRun R,
     K.pass = 0;           // every tile: inside a captured step the kernels run in stream order
     K.dirty = nullptr;
     const int cur = c->cur;
-    const int s = phase == PH_INIT ? 1 : phase;
+    const int s = ph_stage(phase) == PH_INIT ? 1 : ph_stage(phase), attempt = ph_attempt(phase);
     FaceLaunch& f = K.f;
     f.Ub = E->U[(cur + s - 1) & 3];
     f.Uold = E->U[cur & 3];
@@ -167,10 +173,32 @@ __global__ __launch_bounds__((face_threads<TX, TY>())) void kb_faces(BatchRun R,
     f.tau = phase_tau(c, phase);
     f.tau_state = c->tau + c->D;
     f.D = c->D;
-    f.wold = s == 2 ? 0.75 : (s == 1 ? 1.0 : 0.0);
-    f.wnew = s == 2 ? 0.25 : 0.0;
-    f.force = nullptr;
+    // step_ssprk3_2d!: 3/4, 1/4 in stage 2; step_ssprk2_2d!: 1/2, 1/2 (timestepper2d.jl:74-188)
+    f.wold = s == 2 ? (R.integ == FIVO_SSPRK2 ? 0.5 : 0.75) : (s == 1 ? 1.0 : 0.0);
+    f.wnew = s == 2 ? (R.integ == FIVO_SSPRK2 ? 0.5 : 0.25) : 0.0;
+    // a retry takes first-order faces on the mask expanded `attempt` times (kb_expand)
+    f.force = attempt == 0 ? nullptr : ((attempt & 1) ? K.w.force : K.w.ftmp);
     faces_body<EK, NCH, TX, TY, 1>(K, blockIdx.x - E->b0[KB_FACES], sF);
+}
+
+// ---- expand_bad_2d! (src2d/floors2d.jl:369-383): one round of the MOOD mask's growth -------
+// round 0 reads the stage's bad mask; the rounds alternate between force and ftmp,
+// so after `attempt` rounds the mask is in force (odd) or ftmp (even)
+__global__ __launch_bounds__(256) void kb_expand(BatchRun R, int phase, int round) {
+    const int e = block_event(R, KB_CELLS);
+    CCtl c = (CCtl)(R.ctl + e);
+    if (!ev_active(c, phase)) return;
+    CEv E = (CEv)(R.ev + e);
+    const Params p = ev_params(R, E);
+    const int t = (blockIdx.x - E->b0[KB_CELLS]) * 256 + threadIdx.x;
+    if (t >= p.Ntot) return;
+    const uint8_t* in = round == 0 ? E->w.bad : (((round - 1) & 1) == 0 ? E->w.force : E->w.ftmp);
+    uint8_t* out = (round & 1) == 0 ? E->w.force : E->w.ftmp;
+    const int ix = t / p.Nytot, iy = t - ix * p.Nytot;
+    uint8_t v = in[t];
+    if (!v && ix >= p.ng && ix < p.ng + p.Nx && iy >= p.ng && iy < p.ng + p.Ny)
+        v = in[t - 1] | in[t + 1] | in[t - p.Nytot] | in[t + p.Nytot];
+    out[t] = v ? 1 : 0;
 }
 
 // ---- near-bound cells of a stage output: cold inversion ---------------------------------
@@ -182,7 +210,7 @@ __global__ __launch_bounds__(256) void kb_pending(BatchRun R, int phase) {
     CEv E = (CEv)(R.ev + e);
     if (E->sc->npending == 0) return;
     const Params p = ev_params(R, E);
-    const int s = phase == PH_INIT ? 1 : phase;
+    const int s = ph_stage(phase) == PH_INIT ? 1 : ph_stage(phase);
     pending_cell<EK, NCH>(E->U[(c->cur + s) & 3], p, c->tau + c->D, (uint8_t*)E->w.bad, E->sc,
                           (blockIdx.x - E->b0[KB_INTERIOR]) * 256 + threadIdx.x);
 }
@@ -252,25 +280,34 @@ __global__ __launch_bounds__(64) void kb_ctl(BatchRun R, int what) {
             if (c.have_hist || c.nsteps > 0) { c.dohist = 1; c.have_hist = 1; }
         }
         c.stage = 0;
+        c.mood_attempt = 0;
         reset_pass(sc);
         return;
     }
-    if (what >= 11 && what <= 13) {
-        const int s = what - 10;
-        if (!(c.state == EV_RUNNING && c.stage == s - 1)) return;
+    if (what % 100 >= 11 && what % 100 <= 13) {
+        // stage check; what = 10 + stage + 100 attempt
+        const int s = what % 100 - 10, attempt = what / 100, ns = R.integ == FIVO_SSPRK2 ? 2 : 3;
+        if (!(c.state == EV_RUNNING && c.stage == s - 1 && c.mood_attempt == attempt)) return;
         c.nrhs += 1;
         if (sc->nbad == 0) {
             c.stage = s;
-            if (s == 3) c.amax = __longlong_as_double((long long)sc->amax_bits);
+            c.mood_attempt = 0;
+            if (s == ns) c.amax = __longlong_as_double((long long)sc->amax_bits);
             // without relaxation the step's primitive counts are those of its last pass
-            if (s == 3 && !R.dissip) { c.nprimfail += (long long)sc->nprimfail; c.nvacuum += (long long)sc->nvacuum; }
+            if (s == ns && !R.dissip) { c.nprimfail += (long long)sc->nprimfail; c.nvacuum += (long long)sc->nvacuum; }
+        } else if (attempt < R.max_mood) {
+            // _stage_2d! (timestepper2d.jl:113-140): redo the stage with first-order
+            // faces on the bad mask expanded 1 + attempt times
+            c.mood_attempt = attempt + 1;
+            c.nmood += 1;
         } else {
             c.stage = -1;   // attempt failed: no later stage of this replay runs
+            c.mood_attempt = 0;
         }
-        if (s == 3 || c.stage < 0) {
-            // step_ssprk3_2d! outcome: accept, or halve dt and retry next replay
-            if (c.stage == 3) {
-                c.cur = (c.cur + 3) & 3;
+        if (c.stage == ns || c.stage < 0) {
+            // step outcome: accept, or halve dt and retry next replay
+            if (c.stage == ns) {
+                c.cur = (c.cur + ns) & 3;
                 c.tau += c.D;
                 c.nsteps += 1;
                 c.accepted = 1;
@@ -384,6 +421,10 @@ hipError_t launch_batch_faces(Batch& b, int phase, int tile) {
         default: EOS_DISPATCH_B(b, e = (batch_faces_tx<EK, NCH, 15, 16>(b, phase))); break;
     }
     return e;
+}
+hipError_t launch_batch_expand(Batch& b, int phase, int round) {
+    hipLaunchKernelGGL(kb_expand, dim3(b.nblocks[KB_CELLS]), dim3(256), 0, b.stream, b.run, phase, round);
+    return hipGetLastError();
 }
 hipError_t launch_batch_pending(Batch& b, int phase) {
     EOS_DISPATCH_B(b, hipLaunchKernelGGL((kb_pending<EK, NCH>), dim3(b.nblocks[KB_INTERIOR]), dim3(256), 0,

@@ -193,7 +193,8 @@ struct EvCtl {
     int have_hist;
     int accepted;   // the current replay accepted a step
     int dohist;
-    long long nsteps, max_steps, nprimfail, nvacuum, nhalvings, nrhs;
+    int mood_attempt;   // MOOD retry of the current stage being run (0: the first attempt)
+    long long nsteps, max_steps, nprimfail, nvacuum, nhalvings, nrhs, nmood;
 };
 struct BatchScalars {
     unsigned long long nrunning;   // events still running after the last replay
@@ -210,11 +211,14 @@ struct BatchRun {                  // kernel arguments shared by the ensemble ke
     int max_halve;
     int dissip;
     int learned;
+    int integ;                     // FIVO_SSPRK2 or FIVO_SSPRK3
+    int max_mood;                  // MOOD retries per stage (SSPRK2; 0 with SSPRK3)
     MlpDev mlp;
     double cfl, cfltau;
 };
-// phases of one replay of the step graph
-enum { PH_INIT = 0, PH_STAGE1 = 1, PH_STAGE2 = 2, PH_STAGE3 = 3, PH_REFRESH = 4 };
+// phases of one replay of the step graph; a MOOD retry of stage s is phase
+// s + PH_RETRY * attempt (attempt >= 1)
+enum { PH_INIT = 0, PH_STAGE1 = 1, PH_STAGE2 = 2, PH_STAGE3 = 3, PH_REFRESH = 4, PH_RETRY = 8 };
 // host side of an ensemble
 struct Batch {
     std::vector<Ctx*> evs;          // one-event contexts: state, work arrays, scalars
@@ -243,6 +247,7 @@ hipError_t launch_batch_bc(Batch& b, int phase, int role);
 hipError_t launch_batch_prim(Batch& b, int phase);
 hipError_t launch_batch_faces(Batch& b, int phase, int tile);
 hipError_t launch_batch_pending(Batch& b, int phase);
+hipError_t launch_batch_expand(Batch& b, int phase, int round);   // one round of expand_bad_2d!
 hipError_t launch_batch_history(Batch& b);
 hipError_t launch_batch_relax(Batch& b);
 hipError_t launch_batch_project_shear(Batch& b);

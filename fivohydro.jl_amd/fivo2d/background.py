@@ -136,15 +136,15 @@ def export_background2d(ic, out=None, N=200, tau0=0.4, tauf=8.0, nt=39, xmax=16.
 
 
 def export_backgrounds2d(ics, N=200, tau0=0.4, tauf=8.0, nt=39, xmax=16.0, DsT=0.1163, eta_over_s=0.10,
-                         zeta_over_s=0.10, lib=None, CFL=0.2, CFLtau=0.02, learned=None, autotune=True,
-                         verbose=False):
+                         zeta_over_s=0.10, lib=None, CFL=0.2, CFLtau=0.02, integrator="ssprk2", learned=None,
+                         autotune=True, verbose=False):
     """export_background2d for many events at once: every event's chunk between two
     slice times is one ensemble run (one launch per phase over all events, the step
     replayed as a hipGraph, each event with its own dt), then each event's slice is
     written into its own tables in HBM. `ics` is a list of what export_background2d
-    takes as `ic`; `N` one size or one per event. The ensemble integrates with
-    SSPRK3, so each event's tables equal export_background2d(..., integrator="ssprk3")
-    bit for bit. `learned`: an Mlp to seed cold recoveries (results then agree with
+    takes as `ic`; `N` one size or one per event. Each event's tables equal
+    export_background2d(...) with the same integrator (SSPRK2 with MOOD retries, the
+    reference exporter's, by default) bit for bit. `learned`: an Mlp to seed cold recoveries (results then agree with
     the plain path to the learned seed's tolerance, not bit for bit). Returns one
     result dict per event, with the exporter's keys."""
     from .batch import Ensemble
@@ -171,7 +171,8 @@ def export_backgrounds2d(ics, N=200, tau0=0.4, tauf=8.0, nt=39, xmax=16.0, DsT=0
         jmax = lambda a, b: math.nan if (math.isnan(a) or math.isnan(b)) else max(a, b)
         for k, tk in enumerate(ts):
             if tk > min(taus) + 1e-12:
-                rs = ens.run_sim_2d(tau0=taus, taufinal=float(tk), CFL=CFL, CFLtau=CFLtau, reset_history=False)
+                rs = ens.run_sim_2d(tau0=taus, taufinal=float(tk), CFL=CFL, CFLtau=CFLtau, reset_history=False,
+                                    integrator=integrator)
                 for e, r in enumerate(rs):
                     if not r["ok"]:
                         raise FivoError(f"export_backgrounds2d: event {e}'s solve failed at tau = {r['tau']}")

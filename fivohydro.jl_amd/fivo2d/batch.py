@@ -2,9 +2,10 @@
 
 The reference runs fluctuating events one at a time, each through its own
 run_sim_2d! (main2D.jl:371-561). `Ensemble` holds many events on their own grids
-with one model and runs them together: every phase of the SSPRK3 step is one
-launch over all events and the step is a replayed hipGraph, while each event
-keeps its own dt, halvings and stopping time. `ensemble.events[k]` is a
+with one model and runs them together: every phase of the step (SSPRK3, or
+SSPRK2 with its MOOD stage retries) is one launch over all events and the step
+is a replayed hipGraph, while each event keeps its own dt, halvings, retries and
+stopping time. `ensemble.events[k]` is a
 `Solver` over event k's context, for initial conditions and readout.
 """
 import ctypes as C
@@ -73,8 +74,10 @@ class Ensemble:
         self.tile = tuple(tile)
 
     def run_sim_2d(self, tau0=0.4, taufinal=13.0, CFL=0.2, CFLtau=0.05, bc=_abi.FIVO_BC_OUTFLOW,
-                   reset_history=True, max_steps=2_000_000, max_halve=12):
-        """run_sim_2d! (main2D.jl:371-561) for every event with SSPRK3; one result dict per event.
+                   reset_history=True, max_steps=2_000_000, max_halve=12, integrator="ssprk3", max_mood=2):
+        """run_sim_2d! (main2D.jl:371-561) for every event; one result dict per event.
+        integrator "ssprk3" (halving only) or "ssprk2" (MOOD stage retries, up to
+        max_mood, then halving), as the one-event run_sim_2d.
         tau0: one start time, or one per event (to continue events that stopped apart)."""
         n = len(self.grids)
         t0s = None
@@ -84,9 +87,9 @@ class Ensemble:
             tau0 = float(t0s[0])
         o = _abi.RunOpts()
         o.tau0, o.taufinal, o.cfl, o.cfltau, o.dump_dt = tau0, taufinal, CFL, CFLtau, 0.0
-        o.integrator = _abi.FIVO_SSPRK3
+        o.integrator = _abi.FIVO_SSPRK3 if integrator in ("ssprk3", 3) else _abi.FIVO_SSPRK2
         o.bc = int(bc)
-        o.max_halve, o.max_mood, o.max_steps = max_halve, 0, max_steps
+        o.max_halve, o.max_mood, o.max_steps = max_halve, max_mood, max_steps
         o.reset_history = int(bool(reset_history))
         res = (_abi.RunResult * n)()
         tp = None if t0s is None else t0s.ctypes.data_as(C.POINTER(C.c_double))

@@ -288,9 +288,10 @@ int fivo_set_learned_seed(fivo_ctx* ctx, const fivo_mlp* m);
 /* ---- ensembles of events ------------------------------------------------- */
 /* The reference runs fluctuating events one at a time, each with its own
  * run_sim_2d! (main2D.jl:371-561). An ensemble runs many such events at once:
- * each phase of the SSPRK3 step is one launch over every event, the step is a
- * captured hipGraph, and every event keeps its own dt, halvings and stopping
- * time, so each event's result is the one fivo_run gives for it alone. */
+ * each phase of the SSPRK3 or SSPRK2 step (with its MOOD stage retries) is one
+ * launch over every event, the step is a captured hipGraph, and every event
+ * keeps its own dt, halvings, retries and stopping time, so each event's result
+ * is the one fivo_run gives for it alone. */
 typedef struct fivo_batch fivo_batch;
 /* nev events on their own grids, one model (build_model_2d, main2D.jl:85-123) */
 int fivo_batch_create(const fivo_grid_desc* grids, int32_t nev, const fivo_model_desc* m, fivo_batch** out);
@@ -308,7 +309,8 @@ fivo_ctx* fivo_batch_event(fivo_batch* b, int32_t ev);
 int fivo_batch_autotune(fivo_batch* b, double tau, int32_t* tx, int32_t* ty, float* ms);
 /* use candidate tile `tile` (0..FIVO_BATCH_NTILES-1) without timing */
 int fivo_batch_set_tile(fivo_batch* b, int32_t tile);
-/* run_sim_2d! for every event (SSPRK3, no dump callback): res[ev] per event.
+/* run_sim_2d! for every event (SSPRK3, or SSPRK2 with up to 4 MOOD retries per
+ * stage; no dump callback): res[ev] per event.
  * tau0: nullable per-event start times (else o->tau0 for all), e.g. to continue
  * a run whose events stopped at different times. Returns FIVO_STEP_FAILED when
  * some event failed, like fivo_run for it alone. */

@@ -105,7 +105,8 @@ def test_export_at_the_reference_defaults(hip, tmp_path):
 def test_ensemble_export_equals_one_event_exports(hip):
     """export_backgrounds2d (every event advanced by one ensemble run per slice
     interval, slices written per event in HBM) against export_background2d run on
-    each event alone with the same integrator: the same tables, bit for bit (the
+    each event alone, both with the reference exporter's SSPRK2: the same tables, bit
+    for bit (the
     ensemble autotunes its face tile first; tuning restores the warm starts its
     primitive pass moves, so it changes nothing)."""
     ics = [lambda g, s=s: ic.hotspot_event(g, seed=s, nspots=40, sigma=0.6, Tpeak=0.42) for s in (31, 32, 33)]
@@ -113,7 +114,7 @@ def test_ensemble_export_equals_one_event_exports(hip):
     kw = dict(tau0=0.4, tauf=1.0, nt=4, xmax=8.0)
     many = F.export_backgrounds2d(ics, N=Ns, lib=hip, **kw)
     for e, (icf, N) in enumerate(zip(ics, Ns)):
-        one = F.export_background2d(icf, N=N, lib=hip, integrator="ssprk3", verbose=False, **kw)
+        one = F.export_background2d(icf, N=N, lib=hip, verbose=False, **kw)
         assert many[e]["_run"]["steps"] == one["_run"]["steps"], e
         for k in ("T_grid", "vx_grid", "vy_grid", "n_grid", "xgrid", "ygrid", "tgrid"):
             assert np.array_equal(many[e][k], one[k]), (e, k)

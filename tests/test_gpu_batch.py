@@ -181,3 +181,39 @@ def test_autotune_leaves_the_solver_state_unchanged(hip):
             assert np.array_equal(before[k][f], after[f], equal_nan=True), (k, f)
         assert np.array_equal(U0[k], s.get_state()), k
     ens.close()
+
+
+def test_ensemble_ssprk2_with_mood_equals_one_event_runs(hip):
+    """SSPRK2 ensembles (step_ssprk2_2d!: MOOD stage retries on the expanded bad mask,
+    then halving) against each event run alone: the sharp event of the MOOD tests,
+    which retries stages, beside two milder events; the same counts (MOOD retries
+    included) and the same final state, bit for bit."""
+    # the model and the sharp event of the MOOD test (test_oracle_mood.mood_run)
+    m = F.build_model_2d(eos=F.LatticeHRGEOS(), enable_shear=True, eta_over_s=0.08, tauShear_coeff=0.2,
+                         enable_bulk=True, zeta_over_s=0.1, tauPi_coeff=15.0, enable_diff=True,
+                         kappa_coeff=0.1163, pi_clip_factor=1.0)
+    evs = []
+    g = F.make_grid2d(64, 64, xmax=8.0, ymax=8.0)
+    evs.append((g, ic.hotspot_event(g, seed=3, nspots=60, sigma=0.25, Tpeak=0.5)))
+    evs += events()[:2]
+    run = dict(tau0=0.6, taufinal=1.6, CFL=0.6, CFLtau=0.05)
+    ens = F.Ensemble([g for g, _ in evs], m, lib=hip)
+    for s, (g, prims) in zip(ens.events, evs):
+        s.init_from_prims(run["tau0"], **prims)
+    together = ens.run_sim_2d(integrator="ssprk2", max_mood=2, **run)
+    states = [s.get_state() for s in ens.events]
+    ens.close()
+    total_mood = 0
+    for k, (g, prims) in enumerate(evs):
+        s = F.Solver(g, m, lib=hip)
+        s.init_from_prims(run["tau0"], **prims)
+        ra = s.run_sim_2d(integrator="ssprk2", max_mood=2, **run)
+        rb = together[k]
+        print(f"event {k}: alone steps {ra['nsteps']} MOOD {ra['nmood']} halvings {ra['nhalvings']}; "
+              f"ensemble steps {rb['nsteps']} MOOD {rb['nmood']} halvings {rb['nhalvings']}", flush=True)
+        for key in ("ok", "nsteps", "nhalvings", "nmood", "nprimfail", "nvacuum", "nrhs", "tau"):
+            assert ra[key] == rb[key], (k, key, ra[key], rb[key])
+        assert np.array_equal(s.get_state(), states[k]), k
+        total_mood += ra["nmood"]
+        s.close()
+    assert total_mood >= 1
