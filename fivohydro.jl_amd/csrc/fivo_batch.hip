@@ -42,9 +42,9 @@ int build_maps(Batch& b) {
         v.nb[KB_INTERIOR] = nblk((int64_t)p.Nx * p.Ny, 256);
         v.nb[KB_GHOST] = nblk(2 * (int64_t)p.ng * p.Nytot + 2 * (int64_t)p.ng * p.Nx, 256);
         v.nb[KB_FACES] = batch_face_blocks(p, b.tile, &v.tiles_y, &v.nmain);
-        // 8 cells in flight per 64-lane block; one group per queued cell of a warm
-        // pass (the queue holds ~0.5 % of the cells), so no group solves two cells
-        // one after the other; blocks past the queue exit at once
+        // 64 / coop_lanes cells in flight per 64-lane block; one group per queued
+        // cell of a warm pass (the queue holds ~0.5 % of the cells), so no group
+        // solves two cells one after the other; blocks past the queue exit at once
         v.nb[KB_COOP] = std::max(4, std::min(1024, p.Ntot / 64));
         for (int k = 0; k < KB_N; ++k) {
             v.b0[k] = b.nblocks[k];
@@ -197,6 +197,11 @@ int fivo_batch_create(const fivo_grid_desc* grids, int32_t nev, const fivo_model
     }
     // until the autotuner runs: the one-event choice for the mean event size
     b->tile = (cells / nev >= 300 * 300) ? 2 : 0;
+    // Up to ~1M cells the queue of a pass (~0.5 % of the cells) leaves most SIMDs
+    // idle, so the pass is latency-bound and 16 lanes per cell (the Jacobian
+    // evaluated ahead) win; past that the queue fills the GPU and 8 lanes, twice
+    // the cells in flight, win (measured: 64 events of 128-384^2, 32 of 384^2).
+    b->coop_lanes = cells <= ((int64_t)1 << 20) ? 16 : 8;
     b->run.p0 = b->p0;
     b->run.ev = b->d_ev; b->run.ctl = b->d_ctl; b->run.bs = b->bs; b->run.nev = nev;
     int rc = build_maps(*b);

@@ -90,10 +90,10 @@ __global__ __launch_bounds__(256) void kb_prim(BatchRun R, int phase) {
     prim_counts(E->sc, its, nfail, nvac, nq);
 }
 
-// ---- primitive pass: the event's queued cells, GW lanes per cell ------------------------
+// ---- primitive pass: the event's queued cells, G lanes per cell -----------------------------
 // Two wavefronts per SIMD (registers capped at 256, a few spilled): an ensemble's
 // queue holds tens of thousands of cells, so occupancy beats per-wave latency here.
-template <int EK, bool NCH>
+template <int EK, bool NCH, int G>
 __global__ __launch_bounds__(64, 2) void kb_coop(BatchRun R, int phase) {
     const int e = block_event(R, KB_COOP);
     CCtl c = (CCtl)(R.ctl + e);
@@ -101,9 +101,9 @@ __global__ __launch_bounds__(64, 2) void kb_coop(BatchRun R, int phase) {
     CEv E = (CEv)(R.ev + e);
     const int n = (int)E->sc->ncold;
     const int lb = blockIdx.x - E->b0[KB_COOP];
-    const int ngroups = E->nb[KB_COOP] * (64 / GW);
-    int t = (lb * 64 + (int)threadIdx.x) / GW;
-    if (__builtin_amdgcn_readfirstlane(lb * (64 / GW)) >= n) return;
+    const int ngroups = E->nb[KB_COOP] * (64 / G);
+    int t = (lb * 64 + (int)threadIdx.x) / G;
+    if (__builtin_amdgcn_readfirstlane(lb * (64 / G)) >= n) return;
     const Params p = ev_params(R, E);
     const DevWork w = ev_work(E);
     const double* U = E->U[(c->cur + phase_in_role(phase)) & 3];
@@ -112,11 +112,11 @@ __global__ __launch_bounds__(64, 2) void kb_coop(BatchRun R, int phase) {
     unsigned long long its = 0, nfail = 0, nvac = 0, nfb = 0;
     for (; t < n; t += ngroups) {
         int itsum;
-        coop_job<EK, NCH>(U, p, tau, w, jobs[t], itsum, nfail, nvac, nfb);
-        if (glane() == 0) its += itsum;
+        coop_job<EK, NCH, G>(U, p, tau, w, jobs[t], itsum, nfail, nvac, nfb);
+        if (glane<G>() == 0) its += itsum;
     }
     prim_counts(E->sc, its, nfail, nvac);
-    if (glane() == 0 && nfb) atomicAdd(&E->sc->nfallback, nfb);
+    if (glane<G>() == 0 && nfb) atomicAdd(&E->sc->nfallback, nfb);
 }
 
 // ---- learned seeds of the event's queued cold cells, 16 per wavefront ---------------------
@@ -392,8 +392,12 @@ hipError_t launch_batch_prim(Batch& b, int phase) {
     if (b.run.learned)
         EOS_DISPATCH_B(b, hipLaunchKernelGGL((kb_seed<EK, NCH>), dim3(b.nblocks[KB_COOP]), dim3(64), 0, b.stream,
                                              b.run, phase));
-    EOS_DISPATCH_B(b, hipLaunchKernelGGL((kb_coop<EK, NCH>), dim3(b.nblocks[KB_COOP]), dim3(64), 0, b.stream,
-                                         b.run, phase));
+    if (b.coop_lanes == 16)
+        EOS_DISPATCH_B(b, hipLaunchKernelGGL((kb_coop<EK, NCH, 16>), dim3(b.nblocks[KB_COOP]), dim3(64), 0,
+                                             b.stream, b.run, phase));
+    else
+        EOS_DISPATCH_B(b, hipLaunchKernelGGL((kb_coop<EK, NCH, 8>), dim3(b.nblocks[KB_COOP]), dim3(64), 0,
+                                             b.stream, b.run, phase));
     return hipGetLastError();
 }
 template <int EK, bool NCH, int TX, int TY>

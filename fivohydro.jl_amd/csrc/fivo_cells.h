@@ -90,10 +90,16 @@ __device__ __forceinline__ void bc_cell(double* __restrict__ U, const Params& p,
 // the warm-start arrays) and queued; a cell with no warm start, or whose warm
 // solve failed (the fresh-seed retry of cons_to_prim_2d!,
 // src2d/primrec2d.jl:465-486), is queued for the cold seed. k_prim_coop then
-// solves the queue with GW lanes per cell: the slow cells set the pass's
+// solves the queue with a group of lanes per cell: the slow cells set the pass's
 // latency, and cooperating lanes cut each iteration to about two residual
-// evaluations on the critical path instead of nine.
+// evaluations on the critical path instead of nine (one, when the line search's
+// first trial is accepted and its 16-lane group has evaluated the next
+// Jacobian beside it).
 constexpr int PRIM_WARM_ITCAP = 6;
+// lanes per queued cell of a single event: its queue is short, so the pass is
+// latency-bound, and 16 lanes evaluate the next Jacobian ahead (an ensemble
+// picks 8 or 16 by its size, Batch::coop_lanes)
+constexpr int COOP_LANES = 16;
 constexpr int PRIM_JOB_TAIL = 1000;   // job.it = PRIM_JOB_TAIL + iters: only the staged fallback is left
 __device__ __forceinline__ void prim_finalize(const Params& p, const DevWork& w, int i, const PrimRes& r0,
                                               const Diss& d, unsigned long long& nfail, unsigned long long& nvac) {
@@ -170,16 +176,16 @@ __device__ __forceinline__ void prim_cell(const double* __restrict__ U, const Pa
     if (vac || warm) {
         NewtonState s;
         PrimRes r;
-        bool fin = newton_begin<EK, NCH, false, false>(c, yT0, w.x0_phi[i], ux0, uy0, p.eos, s, r);
+        bool fin = newton_begin<EK, NCH, false, 0>(c, yT0, w.x0_phi[i], ux0, uy0, p.eos, s, r);
         bool paused = false;
         if (!fin) {
-            newton_iterate<EK, NCH, false>(s, c, p.eos, PRIM_WARM_ITCAP);
+            newton_iterate<EK, NCH, 0>(s, c, p.eos, PRIM_WARM_ITCAP);
             paused = newton_paused(s);
             if (paused) {
                 job = s.it;
                 w.x0_yT[i] = s.x[0]; w.x0_phi[i] = s.x[1]; w.x0_ux[i] = s.x[2]; w.x0_uy[i] = s.x[3];
             } else {
-                r = newton_finish<EK, NCH, false, false>(s, c, p.eos);
+                r = newton_finish<EK, NCH, 0, false>(s, c, p.eos);
                 if (r.reason == PRR_UNSET) {      // staged phi fallback: done by the group
                     paused = true;
                     job = PRIM_JOB_TAIL + s.iters;
@@ -204,11 +210,11 @@ __device__ __forceinline__ void prim_cell(const double* __restrict__ U, const Pa
 // for the cold seed, so a poor guess costs a bounded detour.
 constexpr int LEARNED_ITCAP = 12;
 
-// one queued cell, solved by the GW lanes of a group (all lanes return the same counts).
+// one queued cell, solved by the G lanes of a group (all lanes return the same counts).
 // Job modes: it > 0 resumes the paused warm solve; it < 0 starts from the learned
 // seed left in the warm-start arrays; it == 0 starts from the cold seed. A
 // failed resumed or learned solve retries from the cold seed.
-template <int EK, bool NCH>
+template <int EK, bool NCH, int G>
 __device__ __forceinline__ void coop_job(const double* __restrict__ U, const Params& p, double tau, const DevWork& w,
                                          const PrimJob& jb, int& itsum, unsigned long long& nfail,
                                          unsigned long long& nvac, unsigned long long& nfallback) {
@@ -229,26 +235,26 @@ __device__ __forceinline__ void coop_job(const double* __restrict__ U, const Par
             newton_set<EK, NCH>(s, c, p.eos);
         } else {
             const bool ls = mode == 2;
-            fin = newton_begin<EK, NCH, true, true>(c, ls ? w.x0_yT[i] : nan, ls ? w.x0_phi[i] : nan,
+            fin = newton_begin<EK, NCH, true, G>(c, ls ? w.x0_yT[i] : nan, ls ? w.x0_phi[i] : nan,
                                                     ls ? w.x0_ux[i] : nan, ls ? w.x0_uy[i] : nan, p.eos, s, r);
         }
         if (!fin) {
             if (mode == 1 && jb.it >= PRIM_JOB_TAIL) s.iters = jb.it - PRIM_JOB_TAIL;   // tail only
-            else newton_iterate<EK, NCH, true>(s, c, p.eos, mode == 2 ? LEARNED_ITCAP : NEWTON_MAXIT);
+            else newton_iterate<EK, NCH, G>(s, c, p.eos, mode == 2 ? LEARNED_ITCAP : NEWTON_MAXIT);
             if (mode == 2 && newton_paused(s)) {         // a learned start that is not close: give it up
                 itsum += LEARNED_ITCAP;
                 nfallback += 1;
                 mode = 0;
                 continue;
             }
-            r = newton_finish<EK, NCH, true>(s, c, p.eos);
+            r = newton_finish<EK, NCH, G>(s, c, p.eos);
         }
         itsum += r.iters;
         if (mode == 0 || r.ok || r.reason == PRR_VACUUM) break;
         if (mode == 2) nfallback += 1;
         mode = 0;
     }
-    if (glane() == 0) prim_finalize(p, w, i, r, c.d, nfail, nvac);
+    if (glane<G>() == 0) prim_finalize(p, w, i, r, c.d, nfail, nvac);
 }
 
 // The learned seed for 16 queued cold cells (jobs [16 tile, 16 tile + 16)) on one

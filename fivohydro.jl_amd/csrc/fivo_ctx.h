@@ -235,6 +235,7 @@ struct Batch {
     int nblocks[KB_N] = {0, 0, 0, 0, 0};
     std::vector<EvView> h_ev;
     int tile = -1;                  // face tile index chosen by the autotuner
+    int coop_lanes = 8;             // lanes per queued cell in kb_coop (8 or 16)
     float tile_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;

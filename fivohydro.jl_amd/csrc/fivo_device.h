@@ -294,17 +294,18 @@ struct Diss { double nux, nuy, Pi, pixx, pixy, piyy, pieta; };
 // ---- primitive recovery (src2d/primrec2d.jl:300-742) --------------------------
 struct PrimRes { double T, mu, ux, uy, n, e, P; int ok, reason, iters; };
 
-// Group-cooperative form: GW consecutive lanes of a wavefront solve one cell.
-// Every lane of a group holds the same iterate; the lanes split the independent
-// evaluations (finite-difference columns, line-search trials, bisection
-// midpoints) and pick results with shuffles, so the iterate sequence is the
-// sequential one, bit for bit.
-constexpr int GW = 8;
-__device__ __forceinline__ int glane() { return threadIdx.x & (GW - 1); }
-__device__ __forceinline__ double gshfl(double v, int src) { return __shfl(v, src, GW); }
-__device__ __forceinline__ unsigned gbits(bool pred) {   // pred of the group's lanes, bit l = lane l
+// Group-cooperative form: G consecutive lanes of a wavefront solve one cell
+// (G = 8 or 16; G = 0 is the one-lane form). Every lane of a group holds the
+// same iterate; the lanes split the independent evaluations (finite-difference
+// columns, line-search trials, bisection midpoints) and pick results with
+// shuffles, so the iterate sequence is the sequential one, bit for bit. With
+// G = 16 the upper eight lanes also evaluate the next iteration's Jacobian
+// columns at the first line-search trial while the lower eight test the trials.
+template <int G> __device__ __forceinline__ int glane() { return threadIdx.x & (G - 1); }
+template <int G> __device__ __forceinline__ double gshfl(double v, int src) { return __shfl(v, src, G); }
+template <int G> __device__ __forceinline__ unsigned gbits(bool pred) {   // pred of the group's lanes, bit l = lane l
     const unsigned long long b = __ballot(pred);
-    return (unsigned)((b >> ((threadIdx.x & 63) & ~(GW - 1))) & ((1u << GW) - 1));
+    return (unsigned)((b >> ((threadIdx.x & 63) & ~(G - 1))) & ((1u << G) - 1));
 }
 __device__ __forceinline__ double sel8(int k, double a0, double a1, double a2, double a3, double a4, double a5,
                                        double a6) {
@@ -317,16 +318,16 @@ __device__ __forceinline__ double sel8(int k, double a0, double a1, double a2, d
 // `nsteps` steps of  mid = (lo+hi)/2; pred(mid) ? hi = mid : lo = mid.
 // COOP: three steps per round; lanes 0..6 take the 7 midpoints of the 3-level
 // tree below [lo, hi] (node n has children 2n+1 for pred true, 2n+2 otherwise).
-template <bool COOP, class Pred>
+template <int G, class Pred>
 __device__ __forceinline__ void bisect(double& lo, double& hi, int nsteps, Pred pred) {
-    if (!COOP) {
+    if constexpr (G == 0) {
         for (int it = 0; it < nsteps; ++it) {
             double mid = 0.5 * (lo + hi);
             if (pred(mid)) hi = mid; else lo = mid;
         }
         return;
     }
-    const int l = glane();
+    const int l = glane<G>();
     for (int done = 0; done < nsteps;) {
         const int L = nsteps - done < 3 ? nsteps - done : 3;
         const double m0 = 0.5 * (lo + hi);
@@ -335,7 +336,7 @@ __device__ __forceinline__ void bisect(double& lo, double& hi, int nsteps, Pred 
         const int nn = (1 << L) - 1;
         bool pl = false;
         if (l < nn) pl = pred(sel8(l, m0, m1, m2, m3, m4, m5, m6));
-        const unsigned b = gbits(pl);
+        const unsigned b = gbits<G>(pl);
         int node = 0;
         for (int lev = 0; lev < L; ++lev) {
             const double mid = sel8(node, m0, m1, m2, m3, m4, m5, m6);
@@ -348,11 +349,11 @@ __device__ __forceinline__ void bisect(double& lo, double& hi, int nsteps, Pred 
 }
 
 // _invert_e_for_T (src2d/primrec2d.jl:300-314): 60 bisection steps on log T
-template <int EK, bool COOP>
+template <int EK, int G>
 __device__ __forceinline__ double invert_e_for_T(double e_target, double phi, const EosP& q) {
     double lo = log(T_MIN), hi = log(T_SOLVE_MAX);
     if (e_target <= 0.0) return lo;
-    bisect<COOP>(lo, hi, 60, [&](double mid) {
+    bisect<G>(lo, hi, 60, [&](double mid) {
         double Tm = exp(mid), P, em;
         eos_Pe_2d<EK>(Tm, q.m + Tm * phi, q, &P, &em);
         return !isf(em) || em > e_target;
@@ -360,7 +361,7 @@ __device__ __forceinline__ double invert_e_for_T(double e_target, double phi, co
     return 0.5 * (lo + hi);
 }
 // seed_from_conserved_2d (src2d/primrec2d.jl:323-347)
-template <int EK, bool COOP>
+template <int EK, int G>
 __device__ __forceinline__ void seed_from_conserved(double E, double Sx, double Sy, double phi, const EosP& q,
                                                  double* yT_o, double* ux_o, double* uy_o) {
     double M = sqrt(Sx * Sx + Sy * Sy);
@@ -371,7 +372,7 @@ __device__ __forceinline__ void seed_from_conserved(double E, double Sx, double 
         v = denom > TINY ? jmin(M / denom, 0.999999) : 0.0;
         double e = denom * (1 - v * v) - P;
         if (e <= 0.0) e = jmax(E, TINY);
-        yT = invert_e_for_T<EK, COOP>(e, phi, q);
+        yT = invert_e_for_T<EK, G>(e, phi, q);
         double T = exp(yT), Pn, en;
         eos_Pe_2d<EK>(T, q.m + T * phi, q, &Pn, &en);
         if (!isf(Pn)) break;
@@ -475,14 +476,14 @@ __device__ __forceinline__ bool solve_gauss(double J[4][4], const double F[4], d
 }
 
 // _solve_phi_bisect (src2d/primrec2d.jl:416-434) at the (clamped) final T
-template <int EK, bool COOP>
+template <int EK, int G>
 __device__ __forceinline__ bool solve_phi_bisect(double D, double ux, double uy, double nux, double nuy,
                                               const TPart& tp, const EosP& q, double* phi_o) {
     double ut = sqrt(1 + ux * ux + uy * uy);
     double target = D - nu_tau(ux, uy, ut, nux, nuy);
     if (!isf(target) || target <= 0.0) { *phi_o = -PHI_CAP; return false; }
     double lo = -PHI_CAP, hi = PHI_CAP, P, nm, e;
-    bisect<COOP>(lo, hi, 200, [&](double mid) {
+    bisect<G>(lo, hi, 200, [&](double mid) {
         double Pb, nb, eb;
         eos_assemble<EK>(tp, q.m + tp.T * mid, q, &Pb, &nb, &eb);
         return isf(nb) && nb * ut > target;
@@ -503,6 +504,10 @@ struct NewtonState {
     double F[4];    // residual at x
     int it, iters, reason;
     bool converged;
+    // G = 16: the Jacobian at x, evaluated ahead by the line search that chose x
+    // (lane 8 + 2k holds column k); valid while jnext, non-finite if jbad
+    double jcol[4];
+    bool jnext, jbad;
 };
 constexpr int NEWTON_MAXIT = 80;
 __device__ __forceinline__ bool newton_paused(const NewtonState& s) {
@@ -532,7 +537,7 @@ __device__ __forceinline__ bool newton_set(NewtonState& s, const CellIn& c, cons
 // warm start or cold seed, first residual. Returns true with the final R when
 // the attempt ends here (vacuum, non-finite start). SEED = false: the caller
 // guarantees a finite warm start or a vacuum state.
-template <int EK, bool NCH, bool SEED, bool COOP>
+template <int EK, bool NCH, bool SEED, int G>
 __device__ __forceinline__ bool newton_begin(const CellIn& c, double yT0, double phi0, double ux0, double uy0,
                                              const EosP& q, NewtonState& s, PrimRes& R) {
     const double yT_lo = log(T_MIN), yT_hi = log(T_SOLVE_MAX);
@@ -549,7 +554,7 @@ __device__ __forceinline__ bool newton_begin(const CellIn& c, double yT0, double
         s.x[3] = jclamp(uy0, -U_CAP, U_CAP);
     } else if (SEED) {
         double sy, sx, syy;
-        seed_from_conserved<EK, COOP>(c.E, c.Sx, c.Sy, s.x[1], q, &sy, &sx, &syy);
+        seed_from_conserved<EK, G>(c.E, c.Sx, c.Sy, s.x[1], q, &sy, &sx, &syy);
         s.x[0] = jclamp(sy, yT_lo, yT_hi); s.x[2] = sx; s.x[3] = syy;
     }
     s.it = 1; s.iters = 0; s.reason = PRR_UNSET; s.converged = false;
@@ -563,11 +568,37 @@ __device__ __forceinline__ bool newton_begin(const CellIn& c, double yT0, double
 }
 
 // finite-difference Jacobian, scaled rows (src2d/primrec2d.jl:605-632)
-template <int EK, bool NCH, bool COOP>
+// Cooperative form: lane l < 8 evaluates column l/2 at x +- h e_k (fd_lane).
+template <bool NCH>
+__device__ __forceinline__ void fd_point(const double x[4], int l, double y[4], double& h, bool& act, int& si) {
+    constexpr int NU = NCH ? 4 : 3;
+    const int k = l >> 1;
+    act = k < NU;
+    si = NCH ? k : (k == 0 ? 0 : k + 1);
+    const double xs = si == 0 ? x[0] : si == 1 ? x[1] : si == 2 ? x[2] : x[3];
+    h = 1e-6 * jmax(1.0, fabs(xs));
+    const double xv = (l & 1) ? xs - h : xs + h;
+    y[0] = si == 0 ? xv : x[0];
+    y[1] = si == 1 ? xv : x[1];
+    y[2] = si == 2 ? xv : x[2];
+    y[3] = si == 3 ? xv : x[3];
+}
+template <int G>
+__device__ __forceinline__ void fd_column(const double Fl[4], double h, const CellIn& c, double col[4]) {
+    // even lane 2k forms column k from its partner's F(x - h e_k)
+    const double inv2h = 1 / (2 * h);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const double Fm = __shfl_xor(Fl[r], 1, G);
+        col[r] = ((Fl[r] - Fm) * inv2h) * c.rs[r];
+    }
+}
+
+template <int EK, bool NCH, int G>
 __device__ __forceinline__ bool newton_jacobian(const NewtonState& s, const CellIn& c, const EosP& q,
                                                 double J[4][4]) {
     constexpr int NU = NCH ? 4 : 3;
-    if (!COOP) {
+    if constexpr (G == 0) {
 #pragma unroll
         for (int k = 0; k < NU; ++k) {
             const int si = NCH ? k : (k == 0 ? 0 : k + 1);
@@ -598,39 +629,25 @@ __device__ __forceinline__ bool newton_jacobian(const NewtonState& s, const Cell
         return true;
     }
     // lane l evaluates column k = l/2 at x + h e_k (l even) or x - h e_k (l odd)
-    const int l = glane();
-    const int k = l >> 1;
-    const bool act = k < NU;
-    const int si = NCH ? k : (k == 0 ? 0 : k + 1);
-    const double xs = si == 0 ? s.x[0] : si == 1 ? s.x[1] : si == 2 ? s.x[2] : s.x[3];
-    const double h = 1e-6 * jmax(1.0, fabs(xs));
-    const double xv = (l & 1) ? xs - h : xs + h;
-    double y[4] = {s.x[0], s.x[1], s.x[2], s.x[3]};
-    y[0] = si == 0 ? xv : y[0];
-    y[1] = si == 1 ? xv : y[1];
-    y[2] = si == 2 ? xv : y[2];
-    y[3] = si == 3 ? xv : y[3];
+    double y[4], h;
+    bool act;
+    int si;
+    fd_point<NCH>(s.x, glane<G>(), y, h, act, si);
     double Tl = s.Tr;
     TPart tl = s.tp;
     if (act && si == 0) { Tl = exp(y[0]); tl = eos_tpart<EK>(jmax(Tl, T_MIN), q); }
     double Fl[4] = {0.0, 0.0, 0.0, 0.0};
     bool okl = true;
     if (act) okl = evalF<EK, NCH>(Fl, y[0], y[1], y[2], y[3], Tl, tl, c, q);
-    if (gbits(!okl) != 0u) return false;
-    // even lane 2k forms column k from its partner's F(x - h e_k)
-    const double inv2h = 1 / (2 * h);
+    if (gbits<G>(!okl) != 0u) return false;
     double col[4];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const double Fm = __shfl_xor(Fl[r], 1, GW);
-        col[r] = ((Fl[r] - Fm) * inv2h) * c.rs[r];
-    }
+    fd_column<G>(Fl, h, c, col);
 #pragma unroll
     for (int kk = 0; kk < NU; ++kk)
 #pragma unroll
         for (int r = 0; r < NU; ++r) {
             const int rr = NCH ? r : r + 1;
-            J[r][kk] = gshfl(col[rr], 2 * kk);
+            J[r][kk] = gshfl<G>(col[rr], 2 * kk);
         }
     return true;
 }
@@ -638,7 +655,7 @@ __device__ __forceinline__ bool newton_jacobian(const NewtonState& s, const Cell
 // the line search of src2d/primrec2d.jl:648-691: trial ls at alpha 2^-ls,
 // accepted at the first trial with a finite residual below (1 - 1e-4 a) nrm.
 // On return x is the accepted trial, or the last one when none is.
-template <int EK, bool NCH, bool COOP>
+template <int EK, bool NCH, int G>
 __device__ __forceinline__ bool newton_linesearch(NewtonState& s, const CellIn& c, const EosP& q,
                                                   const double delta[4], double alpha0, double nrm) {
     constexpr int NU = NCH ? 4 : 3;
@@ -657,7 +674,7 @@ __device__ __forceinline__ bool newton_linesearch(NewtonState& s, const CellIn& 
         t[2] = jclamp(t[2], -U_CAP, U_CAP);
         t[3] = jclamp(t[3], -U_CAP, U_CAP);
     };
-    if (!COOP) {
+    if constexpr (G == 0) {
         double alpha = alpha0;
         for (int ls = 0; ls < 12; ++ls) {
             double t[4];
@@ -676,38 +693,53 @@ __device__ __forceinline__ bool newton_linesearch(NewtonState& s, const CellIn& 
         }
         return false;
     }
-    // trials 0..7 on lanes 0..7, then 8..11 on lanes 0..3
-    const int l = glane();
+    // trials 0..7 on lanes 0..7, then 8..11 on lanes 0..3. G = 16: in the first
+    // round lanes 8..15 evaluate the Jacobian's columns at trial 0 (fd_point at
+    // that point, T and T-part as newton_jacobian takes them there), kept for the
+    // next iteration when trial 0 is the accepted one.
+    const int l = glane<G>();
     for (int round = 0; round < 2; ++round) {
-        const int ls = 8 * round + l;
-        const bool act = ls < 12;
+        const bool ahead = G == 16 && round == 0 && l >= 8;
+        const int ls = ahead ? 0 : 8 * round + l;
+        const bool act = !ahead && ls < 12;
         double alpha = alpha0;
         for (int j = 0; j < ls; ++j) alpha *= 0.5;
         double t[4];
         trial_point(alpha, t);
-        const double Tl = exp(t[0]);
+        double y[4] = {t[0], t[1], t[2], t[3]}, h = 1.0;
+        bool fdact = false;
+        int si = 1;
+        if (ahead) fd_point<NCH>(t, l - 8, y, h, fdact, si);
+        const bool fd0 = fdact && si == 0;
+        const double Tl = exp(y[0]);
         const double Tn = jmax(Tl, T_MIN);
         TPart tl = s.tp;
         double Fl[4] = {0.0, 0.0, 0.0, 0.0};
-        bool accl = false;
-        if (act) {
-            if (Tn != s.tp.T) tl = eos_tpart<EK>(Tn, q);
-            if (evalF<EK, NCH>(Fl, t[0], t[1], t[2], t[3], Tl, tl, c, q)) {
+        bool accl = false, okl = true;
+        if (act || fdact) {
+            if (fd0 || Tn != s.tp.T) tl = eos_tpart<EK>(Tn, q);
+            okl = evalF<EK, NCH>(Fl, y[0], y[1], y[2], y[3], Tl, tl, c, q);
+            if (act && okl) {
                 double trial = 0.0;
 #pragma unroll
                 for (int r = rlo; r < 4; ++r) trial = jmax(trial, fabs(Fl[r] * c.rs[r]));
                 accl = trial < nrm * (1 - 1e-4 * alpha);
             }
         }
-        const unsigned b = gbits(accl);
+        const unsigned b = gbits<G>(accl);
         const bool last = round == 1;
         if (b != 0u || last) {
             const int src = b != 0u ? __builtin_ctz(b) : 3;   // trial 11 is lane 3 of round 1
 #pragma unroll
-            for (int r = 0; r < 4; ++r) { s.x[r] = gshfl(t[r], src); s.F[r] = gshfl(Fl[r], src); }
-            s.Tr = gshfl(Tl, src);
-            s.tp.T = gshfl(tl.T, src); s.tp.Pl = gshfl(tl.Pl, src); s.tp.el = gshfl(tl.el, src);
-            s.tp.K2 = gshfl(tl.K2, src); s.tp.K1 = gshfl(tl.K1, src); s.tp.x = gshfl(tl.x, src);
+            for (int r = 0; r < 4; ++r) { s.x[r] = gshfl<G>(t[r], src); s.F[r] = gshfl<G>(Fl[r], src); }
+            s.Tr = gshfl<G>(Tl, src);
+            s.tp.T = gshfl<G>(tl.T, src); s.tp.Pl = gshfl<G>(tl.Pl, src); s.tp.el = gshfl<G>(tl.el, src);
+            s.tp.K2 = gshfl<G>(tl.K2, src); s.tp.K1 = gshfl<G>(tl.K1, src); s.tp.x = gshfl<G>(tl.x, src);
+            if constexpr (G == 16) {
+                fd_column<G>(Fl, h, c, s.jcol);
+                s.jbad = gbits<G>(fdact && !okl) != 0u;
+                s.jnext = round == 0 && b != 0u && src == 0;
+            }
             return b != 0u;
         }
     }
@@ -715,11 +747,12 @@ __device__ __forceinline__ bool newton_linesearch(NewtonState& s, const CellIn& 
 }
 
 // Newton iterations it .. itcap of _cons_to_prim_attempt_2d! (src2d/primrec2d.jl:591-692)
-template <int EK, bool NCH, bool COOP>
+template <int EK, bool NCH, int G>
 __device__ __forceinline__ void newton_iterate(NewtonState& s, const CellIn& c, const EosP& q, int itcap) {
     constexpr int NU = NCH ? 4 : 3;
     constexpr int rlo = NCH ? 0 : 1;
     constexpr double tol = 1e-12;
+    s.jnext = false;
     for (; s.it <= NEWTON_MAXIT; ++s.it) {
         if (s.it > itcap) return;                    // paused before iteration s.it
         double nrm = 0.0;
@@ -727,7 +760,21 @@ __device__ __forceinline__ void newton_iterate(NewtonState& s, const CellIn& c, 
         for (int r = rlo; r < 4; ++r) nrm = jmax(nrm, fabs(s.F[r] * c.rs[r]));
         if (nrm < tol) { s.converged = true; s.reason = PRR_CONVERGED; s.iters = s.it - 1; return; }
         double J[4][4];
-        if (!newton_jacobian<EK, NCH, COOP>(s, c, q, J)) { s.reason = PRR_NONFINITE_JACOBIAN; return; }
+        bool ahead = false;
+        if constexpr (G == 16) {
+            if (s.jnext) {                  // evaluated by the line search that chose x
+                if (s.jbad) { s.reason = PRR_NONFINITE_JACOBIAN; return; }
+#pragma unroll
+                for (int kk = 0; kk < NU; ++kk)
+#pragma unroll
+                    for (int r = 0; r < NU; ++r) {
+                        const int rr = NCH ? r : r + 1;
+                        J[r][kk] = gshfl<G>(s.jcol[rr], 8 + 2 * kk);
+                    }
+                ahead = true;
+            }
+        }
+        if (!ahead && !newton_jacobian<EK, NCH, G>(s, c, q, J)) { s.reason = PRR_NONFINITE_JACOBIAN; return; }
         double Fact[4], delta[4];
 #pragma unroll
         for (int r = 0; r < NU; ++r) {
@@ -739,7 +786,7 @@ __device__ __forceinline__ void newton_iterate(NewtonState& s, const CellIn& c, 
         double alpha = 1.0;
 #pragma unroll
         for (int k = 0; k < NU; ++k) { double ad = fabs(delta[k]); if (ad > 1.0) alpha = jmin(alpha, 1.0 / ad); }
-        if (!newton_linesearch<EK, NCH, COOP>(s, c, q, delta, alpha, nrm)) {
+        if (!newton_linesearch<EK, NCH, G>(s, c, q, delta, alpha, nrm)) {
             s.reason = PRR_LINESEARCH_FAILED;
             return;
         }
@@ -751,7 +798,7 @@ __device__ __forceinline__ void newton_iterate(NewtonState& s, const CellIn& c, 
 // residual acceptance, staged phi fallback. BISECT = false: returns with
 // reason PRR_UNSET where the staged fallback would start, leaving it to a caller
 // that resumes the tail with BISECT = true from the same iterate.
-template <int EK, bool NCH, bool COOP, bool BISECT = true>
+template <int EK, bool NCH, int G, bool BISECT = true>
 __device__ __forceinline__ PrimRes newton_finish(NewtonState& s, const CellIn& c, const EosP& q) {
     constexpr int rlo = NCH ? 0 : 1;
     constexpr double tol = 1e-12;
@@ -774,7 +821,7 @@ __device__ __forceinline__ PrimRes newton_finish(NewtonState& s, const CellIn& c
         if (NCH && isf(hyd) && hyd < tol) {
             if (!BISECT) { R.reason = PRR_UNSET; R.ok = 0; R.iters = iters; return R; }
             double phib;
-            bool okb = solve_phi_bisect<EK, COOP>(c.D, ux, uy, c.d.nux, c.d.nuy, tf, q, &phib);
+            bool okb = solve_phi_bisect<EK, G>(c.D, ux, uy, c.d.nux, c.d.nuy, tf, q, &phib);
             double ph = jclamp(phib, -PHI_CAP, PHI_CAP);
             mu = q.m + T * ph;
             eos_assemble<EK>(tf, mu, q, &P, &n, &e);
@@ -789,14 +836,14 @@ __device__ __forceinline__ PrimRes newton_finish(NewtonState& s, const CellIn& c
 }
 
 // _cons_to_prim_attempt_2d! (src2d/primrec2d.jl:511-742). NCH: eos_has_charge_2d.
-template <int EK, bool NCH, bool SEED = true, bool COOP = false>
+template <int EK, bool NCH, bool SEED = true, int G = 0>
 __device__ __forceinline__ PrimRes cons_to_prim_attempt(const CellIn& c, double yT0, double phi0, double ux0,
                                                         double uy0, const EosP& q) {
     NewtonState s;
     PrimRes R;
-    if (newton_begin<EK, NCH, SEED, COOP>(c, yT0, phi0, ux0, uy0, q, s, R)) return R;
-    newton_iterate<EK, NCH, COOP>(s, c, q, NEWTON_MAXIT);
-    return newton_finish<EK, NCH, COOP>(s, c, q);
+    if (newton_begin<EK, NCH, SEED, G>(c, yT0, phi0, ux0, uy0, q, s, R)) return R;
+    newton_iterate<EK, NCH, G>(s, c, q, NEWTON_MAXIT);
+    return newton_finish<EK, NCH, G>(s, c, q);
 }
 
 // cons_to_prim_2d! (src2d/primrec2d.jl:465-486): one fresh-seed retry. Written

@@ -27,15 +27,16 @@ template <int EK, bool NCH>
 __global__ __launch_bounds__(256) void k_prim_coop(const double* __restrict__ U, Params p, double tau, DevWork w,
                                                    DevScalars* sc, const PrimJob* __restrict__ jobs) {
     const int n = (int)sc->ncold;
-    const int ngroups = gridDim.x * (blockDim.x / GW);
+    constexpr int G = COOP_LANES;
+    const int ngroups = gridDim.x * (blockDim.x / G);
     unsigned long long its = 0, nfail = 0, nvac = 0, nfb = 0;
-    for (int t = (blockIdx.x * blockDim.x + threadIdx.x) / GW; t < n; t += ngroups) {
+    for (int t = (blockIdx.x * blockDim.x + threadIdx.x) / G; t < n; t += ngroups) {
         int itsum;
-        coop_job<EK, NCH>(U, p, tau, w, jobs[t], itsum, nfail, nvac, nfb);
-        if (glane() == 0) its += itsum;
+        coop_job<EK, NCH, G>(U, p, tau, w, jobs[t], itsum, nfail, nvac, nfb);
+        if (glane<G>() == 0) its += itsum;
     }
     prim_counts(sc, its, nfail, nvac);
-    if (glane() == 0 && nfb) atomicAdd(&sc->nfallback, nfb);
+    if (glane<G>() == 0 && nfb) atomicAdd(&sc->nfallback, nfb);
 }
 
 // learned seeds for the queued cold cells, one wavefront per 16 cells
@@ -250,9 +251,9 @@ hipError_t launch_prim_seed(Ctx& c, const double* U, double tau) {
 }
 hipError_t launch_prim_coop(Ctx& c, const double* U, double tau) {
     const Params& p = c.p;
-    // one group of GW lanes per queued cell; 2048 blocks hold 64k groups, more
-    // than the queue of a warm pass, and stride over the queue of a cold one
-    const int nb = std::min(nblk((int64_t)p.Ntot * GW, 256), 2048);
+    // one group of COOP_LANES lanes per queued cell; 2048 blocks hold 32k groups,
+    // more than the queue of a warm pass, and stride over the queue of a cold one
+    const int nb = std::min(nblk((int64_t)p.Ntot * COOP_LANES, 256), 2048);
     EOS_DISPATCH(c, hipLaunchKernelGGL((k_prim_coop<EK, NCH>), dim3(nb), dim3(256), 0, c.stream, U, p, tau, c.w,
                                        c.sc, c.jobs));
     return hipGetLastError();
