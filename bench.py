@@ -46,11 +46,13 @@ def log(msg):
 # HBM bytes per launch of the dominant kernel measured with rocprofv3 --pmc
 # FETCH_SIZE / WRITE_SIZE, scaled by the same counters on a device copy of known
 # size (profiles/full_sectors_512_kernel_stats.md); keyed by (config, kernel).
+# The figures are those of the serialised launches the HIP events time
+# (cooperative kernel [alone], face kernel [one pass]).
 TRAFFIC = {
-    ("c3", "k_prim_coop"): (2.0 * 1010.0 + 739.9) * 1024,
-    ("c3", "k_prim"): (2.0 * 15255.5 + 33696.9) * 1024,
-    ("c3", "k_faces"): (2.0 * 42036.5 + 34297.5) * 1024,
-    ("c3", "k_relax"): (2.0 * 43614.3 + 65265.6) * 1024,
+    ("c3", "k_prim_coop"): (2.0 * 1021.5 + 457.7) * 1024,
+    ("c3", "k_prim"): (2.0 * 15265.0 + 33812.6) * 1024,
+    ("c3", "k_faces"): (2.0 * 41897.6 + 34276.8) * 1024,
+    ("c3", "k_relax"): (2.0 * 45088.5 + 67114.3) * 1024,
 }
 
 # Workloads: (description, builder). Algorithmic HBM bytes per cell-update and per
