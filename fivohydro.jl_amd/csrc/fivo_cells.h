@@ -92,14 +92,10 @@ __device__ __forceinline__ void bc_cell(double* __restrict__ U, const Params& p,
 // src2d/primrec2d.jl:465-486), is queued for the cold seed. k_prim_coop then
 // solves the queue with a group of lanes per cell: the slow cells set the pass's
 // latency, and cooperating lanes cut each iteration to about two residual
-// evaluations on the critical path instead of nine (one, when the line search's
-// first trial is accepted and its 16-lane group has evaluated the next
-// Jacobian beside it).
+// evaluations on the critical path instead of nine (one, when the line search
+// accepts a trial whose Jacobian the group's upper lanes evaluated beside it:
+// the first trial with 16 lanes, the first three with 32).
 constexpr int PRIM_WARM_ITCAP = 6;
-// lanes per queued cell of a single event: its queue is short, so the pass is
-// latency-bound, and 16 lanes evaluate the next Jacobian ahead (an ensemble
-// picks 8 or 16 by its size, Batch::coop_lanes)
-constexpr int COOP_LANES = 16;
 constexpr int PRIM_JOB_TAIL = 1000;   // job.it = PRIM_JOB_TAIL + iters: only the staged fallback is left
 __device__ __forceinline__ void prim_finalize(const Params& p, const DevWork& w, int i, const PrimRes& r0,
                                               const Diss& d, unsigned long long& nfail, unsigned long long& nvac) {

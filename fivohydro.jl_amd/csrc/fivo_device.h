@@ -305,7 +305,7 @@ template <int G> __device__ __forceinline__ int glane() { return threadIdx.x & (
 template <int G> __device__ __forceinline__ double gshfl(double v, int src) { return __shfl(v, src, G); }
 template <int G> __device__ __forceinline__ unsigned gbits(bool pred) {   // pred of the group's lanes, bit l = lane l
     const unsigned long long b = __ballot(pred);
-    return (unsigned)((b >> ((threadIdx.x & 63) & ~(G - 1))) & ((1u << G) - 1));
+    return (unsigned)((b >> ((threadIdx.x & 63) & ~(G - 1))) & ((1ull << G) - 1));
 }
 __device__ __forceinline__ double sel8(int k, double a0, double a1, double a2, double a3, double a4, double a5,
                                        double a6) {
@@ -508,6 +508,7 @@ struct NewtonState {
     // (lane 8 + 2k holds column k); valid while jnext, non-finite if jbad
     double jcol[4];
     bool jnext, jbad;
+    int jbase;      // lane of column 0
 };
 constexpr int NEWTON_MAXIT = 80;
 __device__ __forceinline__ bool newton_paused(const NewtonState& s) {
@@ -699,8 +700,9 @@ __device__ __forceinline__ bool newton_linesearch(NewtonState& s, const CellIn& 
     // next iteration when trial 0 is the accepted one.
     const int l = glane<G>();
     for (int round = 0; round < 2; ++round) {
-        const bool ahead = G == 16 && round == 0 && l >= 8;
-        const int ls = ahead ? 0 : 8 * round + l;
+        const bool ahead = G >= 16 && round == 0 && l >= 8;
+        const int at = ahead ? (l - 8) >> 3 : 0;   // the trial whose Jacobian lane l evaluates
+        const int ls = ahead ? at : 8 * round + l;
         const bool act = !ahead && ls < 12;
         double alpha = alpha0;
         for (int j = 0; j < ls; ++j) alpha *= 0.5;
@@ -709,7 +711,7 @@ __device__ __forceinline__ bool newton_linesearch(NewtonState& s, const CellIn& 
         double y[4] = {t[0], t[1], t[2], t[3]}, h = 1.0;
         bool fdact = false;
         int si = 1;
-        if (ahead) fd_point<NCH>(t, l - 8, y, h, fdact, si);
+        if (ahead) fd_point<NCH>(t, (l - 8) & 7, y, h, fdact, si);
         const bool fd0 = fdact && si == 0;
         const double Tl = exp(y[0]);
         const double Tn = jmax(Tl, T_MIN);
@@ -735,10 +737,11 @@ __device__ __forceinline__ bool newton_linesearch(NewtonState& s, const CellIn& 
             s.Tr = gshfl<G>(Tl, src);
             s.tp.T = gshfl<G>(tl.T, src); s.tp.Pl = gshfl<G>(tl.Pl, src); s.tp.el = gshfl<G>(tl.el, src);
             s.tp.K2 = gshfl<G>(tl.K2, src); s.tp.K1 = gshfl<G>(tl.K1, src); s.tp.x = gshfl<G>(tl.x, src);
-            if constexpr (G == 16) {
+            if constexpr (G >= 16) {
                 fd_column<G>(Fl, h, c, s.jcol);
-                s.jbad = gbits<G>(fdact && !okl) != 0u;
-                s.jnext = round == 0 && b != 0u && src == 0;
+                s.jbad = gbits<G>(fdact && !okl && at == src) != 0u;
+                s.jnext = round == 0 && b != 0u && src < (G - 8) / 8;
+                s.jbase = 8 + 8 * src;
             }
             return b != 0u;
         }
@@ -761,7 +764,7 @@ __device__ __forceinline__ void newton_iterate(NewtonState& s, const CellIn& c, 
         if (nrm < tol) { s.converged = true; s.reason = PRR_CONVERGED; s.iters = s.it - 1; return; }
         double J[4][4];
         bool ahead = false;
-        if constexpr (G == 16) {
+        if constexpr (G >= 16) {
             if (s.jnext) {                  // evaluated by the line search that chose x
                 if (s.jbad) { s.reason = PRR_NONFINITE_JACOBIAN; return; }
 #pragma unroll
@@ -769,7 +772,7 @@ __device__ __forceinline__ void newton_iterate(NewtonState& s, const CellIn& c, 
 #pragma unroll
                     for (int r = 0; r < NU; ++r) {
                         const int rr = NCH ? r : r + 1;
-                        J[r][kk] = gshfl<G>(s.jcol[rr], 8 + 2 * kk);
+                        J[r][kk] = gshfl<G>(s.jcol[rr], s.jbase + 2 * kk);
                     }
                 ahead = true;
             }

@@ -23,11 +23,10 @@ __global__ __launch_bounds__(256) void k_prim(const double* __restrict__ U, Para
     prim_counts(sc, its, nfail, nvac, nq);
 }
 
-template <int EK, bool NCH>
+template <int EK, bool NCH, int G>
 __global__ __launch_bounds__(256) void k_prim_coop(const double* __restrict__ U, Params p, double tau, DevWork w,
                                                    DevScalars* sc, const PrimJob* __restrict__ jobs) {
     const int n = (int)sc->ncold;
-    constexpr int G = COOP_LANES;
     const int ngroups = gridDim.x * (blockDim.x / G);
     unsigned long long its = 0, nfail = 0, nvac = 0, nfb = 0;
     for (int t = (blockIdx.x * blockDim.x + threadIdx.x) / G; t < n; t += ngroups) {
@@ -251,11 +250,23 @@ hipError_t launch_prim_seed(Ctx& c, const double* U, double tau) {
 }
 hipError_t launch_prim_coop(Ctx& c, const double* U, double tau) {
     const Params& p = c.p;
-    // one group of COOP_LANES lanes per queued cell; 2048 blocks hold 32k groups,
-    // more than the queue of a warm pass, and stride over the queue of a cold one
-    const int nb = std::min(nblk((int64_t)p.Ntot * COOP_LANES, 256), 2048);
-    EOS_DISPATCH(c, hipLaunchKernelGGL((k_prim_coop<EK, NCH>), dim3(nb), dim3(256), 0, c.stream, U, p, tau, c.w,
-                                       c.sc, c.jobs));
+    // one group of G lanes per queued cell. The queue of a warm pass holds ~0.5 %
+    // of the cells: up to 2^19 cells it leaves most SIMDs idle, so the pass is
+    // latency-bound and 32 lanes (the Jacobian evaluated ahead at three trials)
+    // are fastest; larger grids queue enough cells to fill the GPU, where fewer
+    // lanes per cell keep more cells in flight. 2048 blocks hold the queue of a
+    // warm pass and stride over the queue of a cold one.
+    const int G = p.Ntot <= (1 << 19) ? 32 : (p.Ntot <= (1 << 21) ? 16 : 8);
+    const int nb = std::min(nblk((int64_t)p.Ntot * G, 256), 2048);
+    if (G == 32)
+        EOS_DISPATCH(c, hipLaunchKernelGGL((k_prim_coop<EK, NCH, 32>), dim3(nb), dim3(256), 0, c.stream, U, p, tau,
+                                           c.w, c.sc, c.jobs));
+    else if (G == 16)
+        EOS_DISPATCH(c, hipLaunchKernelGGL((k_prim_coop<EK, NCH, 16>), dim3(nb), dim3(256), 0, c.stream, U, p, tau,
+                                           c.w, c.sc, c.jobs));
+    else
+        EOS_DISPATCH(c, hipLaunchKernelGGL((k_prim_coop<EK, NCH, 8>), dim3(nb), dim3(256), 0, c.stream, U, p, tau,
+                                           c.w, c.sc, c.jobs));
     return hipGetLastError();
 }
 hipError_t launch_prim(Ctx& c, const double* U, double tau) {
