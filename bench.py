@@ -49,10 +49,10 @@ def log(msg):
 # The figures are those of the serialised launches the HIP events time
 # (cooperative kernel [alone], face kernel [one pass]).
 TRAFFIC = {
-    ("c3", "k_prim_coop"): (2.0 * 1021.5 + 457.7) * 1024,
-    ("c3", "k_prim"): (2.0 * 15265.0 + 33812.6) * 1024,
-    ("c3", "k_faces"): (2.0 * 41897.6 + 34276.8) * 1024,
-    ("c3", "k_relax"): (2.0 * 45088.5 + 67114.3) * 1024,
+    ("c3", "k_prim_coop"): (2.0 * 1076.2 + 485.2) * 1024,
+    ("c3", "k_prim"): (2.0 * 15272.0 + 33812.5) * 1024,
+    ("c3", "k_faces"): (2.0 * 41906.4 + 34280.4) * 1024,
+    ("c3", "k_relax"): (2.0 * 45060.3 + 67090.8) * 1024,
 }
 
 # Workloads: (description, builder). Algorithmic HBM bytes per cell-update and per
