@@ -91,10 +91,13 @@ __global__ __launch_bounds__(256) void kb_prim(BatchRun R, int phase) {
 }
 
 // ---- primitive pass: the event's queued cells, G lanes per cell -----------------------------
-// Two wavefronts per SIMD (registers capped at 256, a few spilled): an ensemble's
-// queue holds tens of thousands of cells, so occupancy beats per-wave latency here.
+// G = 8 (a large ensemble): two wavefronts per SIMD (registers capped at 256, a few
+// spilled), since its queue holds tens of thousands of cells and occupancy beats
+// per-wave latency. G = 16 (a small one, a short queue): one wavefront per SIMD,
+// so the registers past 256 live in AGPRs instead of scratch (1.4 % on a 512^2
+// event's batched + learned step).
 template <int EK, bool NCH, int G>
-__global__ __launch_bounds__(64, 2) void kb_coop(BatchRun R, int phase) {
+__global__ __launch_bounds__(64, G == 8 ? 2 : 1) void kb_coop(BatchRun R, int phase) {
     const int e = block_event(R, KB_COOP);
     CCtl c = (CCtl)(R.ctl + e);
     if (!ev_active(c, phase)) return;
