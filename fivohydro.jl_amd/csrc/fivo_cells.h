@@ -256,11 +256,24 @@ __device__ __forceinline__ void coop_job(const double* __restrict__ U, const Par
 // The learned seed for 16 queued cold cells (jobs [16 tile, 16 tile + 16)) on one
 // wavefront: features of the cell's conserved state -> the MLP on MFMA tiles ->
 // seed (yT, phi, ux, uy) into the warm-start arrays, job marked learned (it = -1).
-// With the charge unknown (NCH), phi is raised to at least phi_c, where the
-// scaled charge residual n u^tau / max(|D|, 1e-12) is 1e-11: in a charge-free
-// cell any phi below the tolerance floor solves the charge row, but there its
-// Jacobian row vanishes and the solve would stop; from phi_c a few capped
-// steps reach the floor while T and u converge.
+//
+// With the charge unknown (NCH) the seed is used only where the charge row pins
+// phi: the row n u^tau + nu^tau = D is scaled by max(|D|, 1e-12)
+// (src2d/primrec2d.jl:580-583), so below D = 1e-12 its tolerance is an absolute
+// 1e-24 on n u^tau and phi is fixed only to ~1e-24 / (n u^tau) -- in a
+// charge-free cell not at all: the cold solve walks phi down from 0 until
+// n u^tau < 1e-24 and stops wherever that walk crosses the floor. Which phi a
+// solve returns there depends on where it started, and the charge dynamics of
+// the dilute edge (grad alpha, nu, the charge flux of the reconstructed phi)
+// carry that into D and, through the mu n term of the entropy density in the
+// transport coefficients, into T. Such cells keep the reference's cold seed, so
+// they take the reference's path exactly. A seeded cell needs
+// D - |nu| >= max(1e-12, 1e-3 D) (|nu^tau| < |nu|, so n u^tau is at least
+// that): its phi is then fixed to ~1e-9 or better by the relative tolerance.
+// Above that, phi is raised to at least phi_c, where the scaled charge residual
+// is 1e-11, so the Jacobian's charge row does not vanish at the start.
+constexpr double LEARNED_CHARGE_MIN = 1e-12;
+
 template <int EK, bool NCH>
 __device__ __forceinline__ int seed_tile(const double* __restrict__ U, const Params& p, double tau,
                                          const DevWork& w, PrimJob* jobs, int n, int tile, const MlpDev& M) {
@@ -274,8 +287,9 @@ __device__ __forceinline__ int seed_tile(const double* __restrict__ U, const Par
     if (j < n && jobs[j].it == 0) {
         cell = jobs[j].cell;
         const double D = safe_div(U[cell], tau), Sx = U[P_ + cell], Sy = U[2 * P_ + cell], E = U[3 * P_ + cell];
-        if (isf(E) && E > E_VAC && isf(D) && D >= 0.0 && isf(Sx) && isf(Sy)) {
-            const Diss d = load_diss(U, p, cell);
+        const Diss d = load_diss(U, p, cell);
+        const bool charge_ok = !NCH || (D - sqrt(d.nux * d.nux + d.nuy * d.nuy) >= jmax(LEARNED_CHARGE_MIN, 1e-3 * D));
+        if (isf(E) && E > E_VAC && isf(D) && D >= 0.0 && isf(Sx) && isf(Sy) && charge_ok) {
             fr = mlp_frame(Sx, Sy);
             double f[MLP_IN];
             mlp_features(D, Sx, Sy, E, d, fr, f);

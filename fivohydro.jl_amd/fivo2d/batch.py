@@ -46,8 +46,10 @@ class Ensemble:
         if getattr(self, "b", None) is not None and self.b.value:
             for s in self.events:
                 s.ctx = None
+                s._mlp = None
             self.lib.batch_destroy(self.b)
             self.b = None
+        self._mlp = None
 
     def __del__(self):
         try:
@@ -65,8 +67,16 @@ class Ensemble:
         return self.tile, {_abi.BATCH_TILES[k]: ms[k] for k in range(_abi.FIVO_BATCH_NTILES)}
 
     def set_learned_seed(self, mlp):
-        """Start every event's cells without a warm start from `mlp`'s guess (None: the cold seed)."""
+        """Start every event's cells without a warm start from `mlp`'s guess (None: the cold seed).
+        The ensemble keeps a reference to `mlp` while it is attached (see Solver.set_learned_seed)."""
+        if mlp is not None and not (getattr(mlp, "m", None) is not None and mlp.m.value):
+            raise FivoError("set_learned_seed: the network is closed")
         self._chk(self.lib.batch_set_learned_seed(self.b, None if mlp is None else mlp.m), "batch_set_learned_seed")
+        self._mlp = mlp
+        for s in self.events:
+            s._mlp = mlp
+        if mlp is not None:
+            mlp._users.add(self)
 
     def set_tile(self, tile):
         k = _abi.BATCH_TILES.index(tuple(tile))

@@ -9,6 +9,7 @@ start Newton from the network's guess instead of the reference's cold seed
 the fallback. No counterpart in the reference.
 """
 import ctypes as C
+import weakref
 
 import numpy as np
 
@@ -31,6 +32,13 @@ class Mlp:
         if rc != 0 or not m.value:
             raise RuntimeError(f"mlp_create failed ({rc})")
         self.m = m
+        # solvers and ensembles whose contexts point at this network's device
+        # buffers; the network must not be destroyed while any of them holds it
+        self._users = weakref.WeakSet()
+
+    def attached(self):
+        """The solvers and ensembles currently seeded by this network."""
+        return [u for u in list(self._users) if getattr(u, "_mlp", None) is self]
 
     def _chk(self, rc, what):
         if rc < 0:
@@ -39,7 +47,14 @@ class Mlp:
         return rc
 
     def close(self):
+        """Free the network's device buffers. Raises while a solver or ensemble is
+        still seeded by it (their contexts read these buffers in every primitive
+        pass): detach them first with set_learned_seed(None), or close them."""
         if getattr(self, "m", None) is not None and self.m.value:
+            users = self.attached()
+            if users:
+                raise RuntimeError(f"Mlp.close: the network still seeds {len(users)} solver(s)/ensemble(s); "
+                                   "call set_learned_seed(None) on them (or close them) first")
             self.lib.mlp_destroy(self.m)
             self.m = None
 

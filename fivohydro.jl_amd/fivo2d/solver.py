@@ -71,6 +71,7 @@ class Solver:
         if getattr(self, "ctx", None) is not None and self.ctx.value and getattr(self, "_owned", True):
             self.lib.destroy(self.ctx)
         self.ctx = None
+        self._mlp = None
 
     def __del__(self):
         try:
@@ -175,8 +176,15 @@ class Solver:
         return {"nbad": self.init_from_prims(tau0, bc=bc, **prims), "nvacuum": 0}
 
     def set_learned_seed(self, mlp):
-        """Start cells without a warm start from `mlp`'s guess (None: the cold seed)."""
+        """Start cells without a warm start from `mlp`'s guess (None: the cold seed).
+        The solver keeps a reference to `mlp` while it is attached, so the network's
+        device buffers outlive every primitive pass that reads them."""
+        if mlp is not None and not (getattr(mlp, "m", None) is not None and mlp.m.value):
+            raise FivoError("set_learned_seed: the network is closed")
         self._chk(self.lib.set_learned_seed(self.ctx, None if mlp is None else mlp.m), "set_learned_seed")
+        self._mlp = mlp
+        if mlp is not None:
+            mlp._users.add(self)
 
     def finalize_ic(self, tau0, bc=_abi.FIVO_BC_OUTFLOW):
         self._chk(self.lib.finalize_ic(self.ctx, float(tau0), int(bc)), "finalize_ic")
