@@ -896,6 +896,31 @@ int fivo_enable_timing(fivo_ctx* ctx, int on) {
     for (int k = 0; k < TK_N; ++k) { c.t_ms[k] = 0; c.t_cnt[k] = 0; }
     return FIVO_OK;
 }
+static int probe(int which, uint32_t terms, int64_t n, const double* in, double* out, const char* who) {
+    const int nin = which == 0 ? 22 : 44, nout = which == 0 ? 2 : 4;
+    if (n < 0 || (n > 0 && (!in || !out))) return fail(nullptr, FIVO_ERR_ARG, std::string(who) + ": bad arguments");
+    if (terms & ~0x3FFFFu) return fail(nullptr, FIVO_ERR_ARG, std::string(who) + ": terms: unknown bits set");
+    if (n == 0) return FIVO_OK;
+    double *din = nullptr, *dout = nullptr;
+    hipError_t e = hipMalloc(&din, sizeof(double) * nin * n);
+    if (e == hipSuccess) e = hipMalloc(&dout, sizeof(double) * nout * n);
+    if (e == hipSuccess) e = hipMemcpy(din, in, sizeof(double) * nin * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_probe(which, terms, n, din, dout, nullptr);
+    if (e == hipSuccess) e = hipMemcpy(out, dout, sizeof(double) * nout * n, hipMemcpyDeviceToHost);
+    if (din) (void)hipFree(din);
+    if (dout) (void)hipFree(dout);
+    if (e != hipSuccess) return fail(nullptr, FIVO_ERR_HIP, std::string(who) + ": " + hipGetErrorString(e));
+    return FIVO_OK;
+}
+
+int fivo_probe_fm_source(uint32_t terms, int64_t n, const double* in, double* out) {
+    return probe(0, terms, n, in, out, "fivo_probe_fm_source");
+}
+
+int fivo_probe_m2_source(uint32_t terms, int64_t n, const double* in, double* out) {
+    return probe(1, terms, n, in, out, "fivo_probe_m2_source");
+}
+
 int64_t fivo_kernel_time(const fivo_ctx* ctx, int cat, double* total_ms) {
     const Ctx& c = *reinterpret_cast<const Ctx*>(ctx);
     if (cat < 0 || cat >= TK_N) return -1;

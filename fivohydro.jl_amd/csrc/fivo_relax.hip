@@ -164,6 +164,30 @@ __device__ __forceinline__ void rank1_traceless(double Ax, double Ay, double Bx,
     q[1] = (Ay * Bx + Ax * By) / 2 - (ux * uy) * tr / 3;
 }
 
+// consistent_fm_source_2d (src2d/hq_consistent_firstmoment2d.jl:218-282): the
+// first moment's sources (s^x, s^y) in the row's source-on-the-left convention;
+// the relaxation subtracts them from the Navier-Stokes target.
+__device__ __forceinline__ void fm_source(const Params& p, double ux, double uy, double ut, double T, double dxT,
+                                          double dyT, double dtT, double nux, double nuy, double n, double dndT,
+                                          double taun, double Ds, double hp, const Kin& k, double* sx, double* sy) {
+    const double Tm2 = jmax(T, T_MIN);
+    const double DT = ut * dtT + ux * dxT + uy * dyT;
+    const double gTx = dxT + ux * DT, gTy = dyT + uy * DT;
+    const double nut = nu_tau(ux, uy, ut, nux, nuy);
+    const double dtux = (k.ax - ux * k.dxux - uy * k.dyux) * safe_inv(ut);
+    const double dtuy = (k.ay - ux * k.dxuy - uy * k.dyuy) * safe_inv(ut);
+    const double ngux = nut * dtux + nux * k.dxux + nuy * k.dyux;
+    const double nguy = nut * dtuy + nux * k.dxuy + nuy * k.dyuy;
+    const double expansion = term(p, FIVO_T_FM_EXPANSION) ? taun * k.th : 0.0;
+    const double dlnh = term(p, FIVO_T_FM_DLNH) ? (Ds / Tm2) * hp * DT : 0.0;
+    const double br = expansion + dlnh;
+    const double pref = (Ds / Tm2) * (n + Tm2 * dndT);
+    *sx = (term(p, FIVO_T_FM_GRADT) ? pref * gTx : 0.0) + (term(p, FIVO_T_FM_INERTIAL) ? taun * n * k.ax : 0.0) +
+          (term(p, FIVO_T_FM_NU_GRADU) ? taun * ngux : 0.0) + nux * br;
+    *sy = (term(p, FIVO_T_FM_GRADT) ? pref * gTy : 0.0) + (term(p, FIVO_T_FM_INERTIAL) ? taun * n * k.ay : 0.0) +
+          (term(p, FIVO_T_FM_NU_GRADU) ? taun * nguy : 0.0) + nuy * br;
+}
+
 // everything the charm second-moment source takes besides the four fields
 struct M2In {
     double ux, uy, ut, tau, T, dxT, dyT, dtT, nx, ny, peta;
@@ -173,6 +197,19 @@ struct M2In {
     // hq_m2_coeffs_2d (src2d/hq_consistent_m2_2d.jl:125-146), field independent
     double dlnC, etabar, lam_a, Aco, ABr;
 };
+
+// hq_m2_coeffs_2d (src2d/hq_consistent_m2_2d.jl:125-146): the field-independent
+// coefficients, from a.taun, a.n, a.Ds, a.h and the scaled K2..K4 at z = m / Tm
+__device__ __forceinline__ void m2_coeffs(M2In& a, double Tm, double z, double K2, double K3, double K4, double m) {
+    const double K5 = K3 + 8 * K4 / z;
+    const double dlnC = (fabs(K4) > TINY && fabs(K3) > TINY) ? z * ((K3 + K5) / (2 * K4) - (K2 + K4) / (2 * K3)) : 0.0;
+    a.dlnC = isf(dlnC) ? dlnC : 0.0;
+    a.etabar = a.taun * a.n * Tm / 2;
+    a.lam_a = a.Ds * ((m * m) + 6 * Tm * a.h) / (2 * Tm);
+    a.Aco = (m * m) + 5 * Tm * a.h;
+    const double ABr = (fabs(K3) > TINY) ? 5 + z * K2 / K3 : 5.0;
+    a.ABr = isf(ABr) ? ABr : 5.0;
+}
 
 // consistent_m2_source_2d (src2d/hq_consistent_m2_2d.jl:247-427): the four rates.
 // Out of line: the update evaluates it five times.
@@ -430,24 +467,8 @@ __device__ __forceinline__ void relax_cell(const double* __restrict__ Q, double*
                         }
                     }
                     const double Ds = safe_div(p.kappa_coeff, T) / FMGEV;
-                    const double Tm2 = jmax(T, T_MIN);
-                    const double DT = ut * dtT + ux * dxT + uy * dyT;
-                    const double gTx = dxT + ux * DT, gTy = dyT + uy * DT;
-                    const double nut = nu_tau(ux, uy, ut, nux, nuy);
-                    const double dtux = (k.ax - ux * k.dxux - uy * k.dyux) * safe_inv(ut);
-                    const double dtuy = (k.ay - ux * k.dxuy - uy * k.dyuy) * safe_inv(ut);
-                    const double ngux = nut * dtux + nux * k.dxux + nuy * k.dyux;
-                    const double nguy = nut * dtuy + nux * k.dxuy + nuy * k.dyuy;
-                    const double expansion = term(p, FIVO_T_FM_EXPANSION) ? taun * k.th : 0.0;
-                    const double dlnh = term(p, FIVO_T_FM_DLNH) ? (Ds / Tm2) * hp * DT : 0.0;
-                    const double br = expansion + dlnh;
-                    const double pref = (Ds / Tm2) * (n + Tm2 * dndT);
-                    const double csx = (term(p, FIVO_T_FM_GRADT) ? pref * gTx : 0.0) +
-                                       (term(p, FIVO_T_FM_INERTIAL) ? taun * n * k.ax : 0.0) +
-                                       (term(p, FIVO_T_FM_NU_GRADU) ? taun * ngux : 0.0) + nux * br;
-                    const double csy = (term(p, FIVO_T_FM_GRADT) ? pref * gTy : 0.0) +
-                                       (term(p, FIVO_T_FM_INERTIAL) ? taun * n * k.ay : 0.0) +
-                                       (term(p, FIVO_T_FM_NU_GRADU) ? taun * nguy : 0.0) + nuy * br;
+                    double csx, csy;
+                    fm_source(p, ux, uy, ut, T, dxT, dyT, dtT, nux, nuy, n, dndT, taun, Ds, hp, k, &csx, &csy);
                     nsx -= csx; nsy -= csy;
                 }
                 nsx *= wv; nsy *= wv;
@@ -503,15 +524,7 @@ __device__ __forceinline__ void relax_cell(const double* __restrict__ Q, double*
                         a.nx = nux; a.ny = nuy; a.peta = pq[3];
                         a.dta = dta; a.dxa = dxa; a.dya = dya;
                         a.n = n; a.taun = taun; a.tauM = tMq; a.etaM = jmax(T, T_MIN) * taun / 2;
-                        // hq_m2_coeffs_2d (src2d/hq_consistent_m2_2d.jl:125-146)
-                        const double K5 = K3 + 8 * K4 / z;
-                        double dlnC = (fabs(K4) > TINY && fabs(K3) > TINY) ? z * ((K3 + K5) / (2 * K4) - (K2 + K4) / (2 * K3)) : 0.0;
-                        a.dlnC = isf(dlnC) ? dlnC : 0.0;
-                        a.etabar = taun * n * Tm3 / 2;
-                        a.lam_a = a.Ds * ((mq * mq) + 6 * Tm3 * h) / (2 * Tm3);
-                        a.Aco = (mq * mq) + 5 * Tm3 * h;
-                        const double ABr = (fabs(K3) > TINY) ? 5 + z * K2 / K3 : 5.0;
-                        a.ABr = isf(ABr) ? ABr : 5.0;
+                        m2_coeffs(a, Tm3, z, K2, K3, K4, mq);
                         double s0[4], sp[4], cc[4];
                         m2_source(p, a, pq[0], pq[1], pq[2], pq[4], s0);
                         const double A2 = safe_div(tMq * ut, D);
@@ -592,6 +605,63 @@ hipError_t launch_batch_relax(Batch& b) {
         hipLaunchKernelGGL((kb_relax<2>), dim3(b.nblocks[KB_CELLS]), dim3(256), 0, b.stream, b.run);
     else
         hipLaunchKernelGGL((kb_relax<0>), dim3(b.nblocks[KB_CELLS]), dim3(256), 0, b.stream, b.run);
+    return hipGetLastError();
+}
+
+// ---- probes: the relaxation's source terms at arbitrary states ----------------------
+// One thread per input row, arguments in the reference functions' own order, so a
+// test can hand the device code the states the reference's gates use.
+constexpr int PROBE_FM_IN = 22, PROBE_M2_IN = 44;
+
+// consistent_fm_source_2d(ux, uy, ut, T, dxT, dyT, dtT, nux, nuy, n, dn_dT, taun, Ds,
+// h, hp, theta, ax, ay, dxux, dxuy, dyux, dyuy) -> (s^x, s^y)
+__global__ void k_probe_fm(Params p, int64_t n, const double* __restrict__ in, double* __restrict__ out) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const double* a = in + r * PROBE_FM_IN;
+    Kin k;
+    k.th = a[15]; k.ax = a[16]; k.ay = a[17]; k.dxux = a[18]; k.dxuy = a[19]; k.dyux = a[20]; k.dyuy = a[21];
+    k.at = (a[0] * k.ax + a[1] * k.ay) * safe_inv(a[2]);
+    fm_source(p, a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], a[8], a[9], a[10], a[11], a[12], a[14], k,
+              out + 2 * r, out + 2 * r + 1);
+}
+
+// consistent_m2_source_2d(ux, uy, ut, tau, T, dxT, dyT, dtT, nux, nuy, pxx, pxy, pyy,
+// peta, PiQ, dta, dxa, dya, theta_nu, anux, anuy, dtnux, dtnuy, dxnux, dxnuy, dynux,
+// dynuy, theta, ax, ay, dtux, dtuy, dxux, dxuy, dyux, dyuy, n, taun, Ds, h, hp, tauM,
+// etaM, m) -> (dpxx, dpxy, dpyy, dPiQ); the coefficients come from m2_coeffs as in
+// the relaxation
+__global__ void k_probe_m2(Params p, int64_t n, const double* __restrict__ in, double* __restrict__ out) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const double* v = in + r * PROBE_M2_IN;
+    M2In a;
+    a.ux = v[0]; a.uy = v[1]; a.ut = v[2]; a.tau = v[3]; a.T = v[4]; a.dxT = v[5]; a.dyT = v[6]; a.dtT = v[7];
+    a.nx = v[8]; a.ny = v[9]; a.peta = v[13];
+    a.dta = v[15]; a.dxa = v[16]; a.dya = v[17]; a.thnu = v[18];
+    a.dtnx = v[21]; a.dtny = v[22]; a.dxnx = v[23]; a.dxny = v[24]; a.dynx = v[25]; a.dyny = v[26];
+    a.k.th = v[27]; a.k.ax = v[28]; a.k.ay = v[29];
+    a.dtux = v[30]; a.dtuy = v[31];
+    a.k.dxux = v[32]; a.k.dxuy = v[33]; a.k.dyux = v[34]; a.k.dyuy = v[35];
+    a.k.at = (a.ux * a.k.ax + a.uy * a.k.ay) * safe_inv(a.ut);
+    a.n = v[36]; a.taun = v[37]; a.Ds = v[38]; a.h = v[39]; a.hp = v[40]; a.tauM = v[41]; a.etaM = v[42];
+    const double m = v[43];
+    double* o = out + 4 * r;
+    o[0] = o[1] = o[2] = o[3] = 0.0;
+    if (!(a.tauM > 0.0)) return;
+    const double Tm = jmax(a.T, T_MIN), z = m / Tm;
+    m2_coeffs(a, Tm, z, safe_besselkx(2, z), safe_besselkx(3, z), safe_besselkx(4, z), m);
+    m2_source(p, a, v[10], v[11], v[12], v[14], o);
+}
+
+hipError_t launch_probe(int which, uint32_t terms, int64_t n, const double* in, double* out, hipStream_t s) {
+    Params p{};
+    p.terms = terms;
+    const int nb = (int)((n + 63) / 64);
+    if (which == 0)
+        hipLaunchKernelGGL(k_probe_fm, dim3(nb), dim3(64), 0, s, p, n, in, out);
+    else
+        hipLaunchKernelGGL(k_probe_m2, dim3(nb), dim3(64), 0, s, p, n, in, out);
     return hipGetLastError();
 }
 

@@ -136,6 +136,8 @@ ABI_FUNCS = (
     ("relax", C.c_int, [_VP, C.c_double, C.c_double]),
     ("project_shear", C.c_int, [_VP, _DP, _DP]),
     ("run", C.c_int, [_VP, C.POINTER(RunOpts), C.POINTER(RunResult)]),
+    ("probe_fm_source", C.c_int, [C.c_uint32, C.c_int64, _DP, _DP]),
+    ("probe_m2_source", C.c_int, [C.c_uint32, C.c_int64, _DP, _DP]),
 )
 
 

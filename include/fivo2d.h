@@ -246,6 +246,26 @@ int fivo_background_slice(fivo_ctx* ctx, int32_t k, int32_t nt, double* T, doubl
 int64_t fivo_kernel_time(const fivo_ctx* ctx, int category, double* total_ms);
 int fivo_enable_timing(fivo_ctx* ctx, int on);
 
+/* ---- probes of the relaxation's source terms ---------------------------------- */
+/* Evaluate the device code of a source term on n states given row by row (host
+ * arrays, row-major), with the term switches `terms` (FIVO_T_* bits). These exist so
+ * that tests can run the reference's fixed-state gates (1-D limit, rotation
+ * covariance, tracelessness, Bjorken limit) on the HIP path; no solver call uses them.
+ *
+ * fivo_probe_fm_source replaces nothing: it evaluates consistent_fm_source_2d
+ * (src2d/hq_consistent_firstmoment2d.jl:218-282) with its 22 arguments in the
+ * reference's order (ux, uy, ut, T, dxT, dyT, dtT, nux, nuy, n, dn_dT, tau_n, D_s, h,
+ * h', theta, ax, ay, dxux, dxuy, dyux, dyuy) -> (s^x, s^y).
+ * fivo_probe_m2_source evaluates consistent_m2_source_2d
+ * (src2d/hq_consistent_m2_2d.jl:247-427) with its 44 arguments in the reference's
+ * order (ux, uy, ut, tau, T, dxT, dyT, dtT, nux, nuy, pixx_Q, pixy_Q, piyy_Q, pieta_Q,
+ * Pi_Q, dt_alpha, dx_alpha, dy_alpha, theta_nu, a_nu^x, a_nu^y, dt_nux, dt_nuy,
+ * dx_nux, dx_nuy, dy_nux, dy_nuy, theta, ax, ay, dt_ux, dt_uy, dxux, dxuy, dyux, dyuy,
+ * n, tau_n, D_s, h, h', tau_M, eta_M, m) -> the rates (pixx_Q, pixy_Q, piyy_Q, Pi_Q).
+ * No counterpart in the reference (its gates call the Julia functions directly). */
+int fivo_probe_fm_source(uint32_t terms, int64_t n, const double* in, double* out);
+int fivo_probe_m2_source(uint32_t terms, int64_t n, const double* in, double* out);
+
 /* ---- learned Newton warm start --------------------------------------------- */
 /* A 6 -> 64 -> 64 -> 6 GELU MLP (BF16 MFMA, FP32 accumulate) mapping features of a
  * cell's conserved state (log E, log(D + 1e-30), |S|/E, Pi/E, s.pi.s/E,

@@ -599,4 +599,53 @@ double orc_diff_tauN(const fivo_eos_desc* e, double T, double alpha, double DsT,
 void orc_hq_h_hprime(const fivo_eos_desc* e, double T, double* out2) {
     orc::hq_h_hprime(T, orc::EOS(*e), &out2[0], &out2[1]);
 }
+
+// consistent_fm_source_2d (src2d/hq_consistent_firstmoment2d.jl:218-282) on n
+// states, 22 arguments per row in the reference's order -> (s^x, s^y)
+int orc_probe_fm_source(uint32_t terms, int64_t n, const double* in, double* out) {
+    orc::Model md{};
+    md.d.terms = terms;
+    for (int64_t r = 0; r < n; ++r) {
+        const double* a = in + 22 * r;
+        orc::Kin k;
+        k.th = a[15]; k.ax = a[16]; k.ay = a[17]; k.dxux = a[18]; k.dxuy = a[19]; k.dyux = a[20]; k.dyuy = a[21];
+        k.at = (a[0] * k.ax + a[1] * k.ay) * orc::safe_inv(a[2]);
+        orc::consistent_fm_source(md, a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], a[8], a[9], a[10], a[11],
+                                  a[12], a[14], k, &out[2 * r], &out[2 * r + 1]);
+    }
+    return 0;
+}
+
+// consistent_m2_source_2d (src2d/hq_consistent_m2_2d.jl:247-427) on n states, 44
+// arguments per row in the reference's order -> (dpxx, dpxy, dpyy, dPiQ)
+int orc_probe_m2_source(uint32_t terms, int64_t n, const double* in, double* out) {
+    orc::Model md{};
+    md.d.terms = terms;
+    for (int64_t r = 0; r < n; ++r) {
+        const double* v = in + 44 * r;
+        orc::M2In a;
+        a.ux = v[0]; a.uy = v[1]; a.ut = v[2]; a.tau = v[3]; a.T = v[4]; a.dxT = v[5]; a.dyT = v[6]; a.dtT = v[7];
+        a.nx = v[8]; a.ny = v[9]; a.peta = v[13];
+        a.dta = v[15]; a.dxa = v[16]; a.dya = v[17]; a.thnu = v[18]; a.anx = v[19]; a.any = v[20];
+        a.dtnx = v[21]; a.dtny = v[22]; a.dxnx = v[23]; a.dxny = v[24]; a.dynx = v[25]; a.dyny = v[26];
+        a.k.th = v[27]; a.k.ax = v[28]; a.k.ay = v[29];
+        a.dtux = v[30]; a.dtuy = v[31];
+        a.k.dxux = v[32]; a.k.dxuy = v[33]; a.k.dyux = v[34]; a.k.dyuy = v[35];
+        a.k.at = (a.ux * a.k.ax + a.uy * a.k.ay) * orc::safe_inv(a.ut);
+        a.n = v[36]; a.taun = v[37]; a.Ds = v[38]; a.h = v[39]; a.hp = v[40]; a.tauM = v[41]; a.etaM = v[42];
+        a.m = v[43];
+        orc::m2_source(md, a, v[10], v[11], v[12], v[14], out + 4 * r);
+    }
+    return 0;
+}
+
+// project_shear_traceless_2d's pi^eta_eta (src2d/shear2d.jl:131-143) on n states
+// (ux, uy, ut, pixx, pixy, piyy) -> pi^eta_eta = pi^tautau - pixx - piyy
+void orc_probe_pieta(int64_t n, const double* in, double* out) {
+    for (int64_t r = 0; r < n; ++r) {
+        const double* a = in + 6 * r;
+        orc::Pi4 P = orc::shear_tau_row(a[0], a[1], a[2], a[3], a[4], a[5]);
+        out[r] = P.tt - a[3] - a[5];
+    }
+}
 }
