@@ -249,6 +249,41 @@ def background_bench(lib, s, g):
     return out
 
 
+def conditioning_check(lib, e, evs, m, grids, leg, steps, warmup, Tref, OKref, flagged):
+    """The events whose batched+learned run leaves 1e-9 of the plain run (or flips
+    a recovery flag), rerun on the oracle from the same initial state and steps:
+    an event whose plain HIP run and oracle run (round-off apart) also part by more
+    than 1e-9 or flip a flag is perturbation-sensitive in the reference algorithm
+    itself, the learned seed merely one more perturbation."""
+    import fivo2d as F
+    orc = F.Lib(os.path.join(REPO, "oracle", "liboracle2d.so"), "orc_")
+    orc.dll.orc_set_threads.restype = C.c_int
+    orc.dll.orc_set_threads.argtypes = [C.c_int]
+    orc.dll.orc_set_threads(os.cpu_count() or 1)
+    sensitive = []
+    t0 = time.perf_counter()
+    for k in flagged:
+        n, nsp, seed = evs[k]
+        sh = F.Solver(grids[k], m, lib=lib)
+        sh.init_hotspots(e["tau0"], seed=seed, nspots=nsp, sigma=e["sigma"], Tpeak=0.45)
+        so = F.Solver(grids[k], m, lib=orc)
+        so.set_state(sh.get_state())
+        so.set_work(sh.get_work())
+        sh.close()
+        r = so.run_sim_2d(tau0=e["tau0"], max_steps=warmup, integrator="ssprk3", **leg)
+        so.run_sim_2d(tau0=r["tau"], max_steps=steps, integrator="ssprk3", reset_history=False, **leg)
+        w = so.get_work(("yT", "ok"))
+        so.close()
+        both = (w["ok"] == 1) & (OKref[k] == 1)
+        d = float(np.max(np.where(both, np.abs(np.exp(w["yT"]) - Tref[k]) / Tref[k], 0.0)))
+        fl = int(np.sum(w["ok"] != OKref[k]))
+        sensitive.append({"event": k, "plain_vs_oracle_max_dT_over_T": d, "plain_vs_oracle_flips": fl,
+                          "sensitive": bool(d > 1e-9 or fl > 0)})
+    return {"flagged_events": sensitive,
+            "flagged_events_not_sensitive": sum(1 for x in sensitive if not x["sensitive"]),
+            "conditioning_check_s": time.perf_counter() - t0}
+
+
 def ensemble_bench(lib, name, steps, warmup, net):
     """One ensemble three ways on the same events and steps: plain (each event
     through its own run_sim_2d!, one after another), batched (all events per
@@ -317,6 +352,8 @@ def ensemble_bench(lib, name, steps, warmup, net):
         if path == "batched_learned":
             info["learned_starts"] = nl
             info["fallback_rate"] = sum(r["nlearned_fallback"] for r in res) / max(nl, 1)
+            info.update(conditioning_check(lib, e, evs, m, grids, leg, steps, warmup, Tref, OKref,
+                                           [k for k, b in enumerate(bad) if b]))
         out[path] = info
         log(f"{name} {path}: {info['cell_updates_per_s']:.3e} cell-updates/s, max dT/T {dT:.1e}")
         x.close()
@@ -596,6 +633,7 @@ def main():
             tg[f"ensemble_{e}_max_dT_over_T"] = bl["max_dT_over_T_vs_plain"]
             tg[f"ensemble_{e}_events_with_dT_over_1e-9_or_flips"] = bl["events_with_cells_dT_over_1e-9_or_flips"]
             tg[f"ensemble_{e}_max_dT_over_T_other_events"] = bl["max_dT_over_T_other_events"]
+            tg[f"ensemble_{e}_flagged_events_not_perturbation_sensitive"] = bl["flagged_events_not_sensitive"]
     out["targets"] = tg
     if "background" in legs:
         out["background_export"] = background_bench(lib, s, g)

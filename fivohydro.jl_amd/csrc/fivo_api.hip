@@ -696,6 +696,8 @@ int fivo_rhs(fivo_ctx* ctx, double tau, int bc, const uint8_t* force, double* dU
         HIPCHK(&c, hipMemcpyAsync(c.w.force, force, c.p.Ntot, hipMemcpyHostToDevice, c.stream));
         fdev = c.w.force;
     }
+    // this call's retries and staged fallbacks
+    HIPCHK(&c, hipMemsetAsync(&c.sc->nretry, 0, 2 * sizeof(unsigned long long), c.stream));
     int rc = rhs_dev(c, tau, bc, fdev);
     if (rc < 0) return rc;
     if (dU) {
@@ -704,7 +706,10 @@ int fivo_rhs(fivo_ctx* ctx, double tau, int bc, const uint8_t* force, double* dU
         HIPCHK(&c, hipMemcpyAsync(dU, c.snap, sizeof(double) * c.p.nv * c.p.Ntot, hipMemcpyDeviceToHost, c.stream));
         HIPCHK(&c, hipStreamSynchronize(c.stream));
     }
-    if (cnt) { cnt->nprimfail = c.last_primfail; cnt->nvacuum = c.last_vacuum; cnt->amax = c.amax_host; }
+    if (cnt) {
+        cnt->nprimfail = c.last_primfail; cnt->nvacuum = c.last_vacuum; cnt->amax = c.amax_host;
+        cnt->nretry = (int64_t)c.sc_host->nretry; cnt->nstaged = (int64_t)c.sc_host->nstaged;
+    }
     return FIVO_OK;
 }
 
@@ -712,13 +717,17 @@ int fivo_update_primitives(fivo_ctx* ctx, double tau, int bc, fivo_counts* cnt) 
     Ctx& c = *reinterpret_cast<Ctx*>(ctx);
     int rc;
     if ((rc = reset_scalars(c)) < 0) return rc;
+    HIPCHK(&c, hipMemsetAsync(&c.sc->nretry, 0, 2 * sizeof(unsigned long long), c.stream));
     // keep amax: update_primitives_2d! does not touch the signal speeds
     if ((rc = do_bc(c, c.U[c.cur], bc)) < 0) return rc;
     if ((rc = do_prim(c, c.U[c.cur], tau)) < 0) return rc;
     double amax = c.amax_host;
     if ((rc = read_scalars(c)) < 0) return rc;
     c.amax_host = amax;
-    if (cnt) { cnt->nprimfail = c.last_primfail; cnt->nvacuum = c.last_vacuum; cnt->amax = c.amax_host; }
+    if (cnt) {
+        cnt->nprimfail = c.last_primfail; cnt->nvacuum = c.last_vacuum; cnt->amax = c.amax_host;
+        cnt->nretry = (int64_t)c.sc_host->nretry; cnt->nstaged = (int64_t)c.sc_host->nstaged;
+    }
     return FIVO_OK;
 }
 
@@ -844,6 +853,8 @@ int fivo_run(fivo_ctx* ctx, const fivo_run_opts* o, fivo_run_result* r) {
             r->nlearned = (int64_t)c.sc_host->nlearned;
             r->nlearned_fallback = (int64_t)c.sc_host->nfallback;
             r->nqueued = (int64_t)c.sc_host->nqueued;
+            r->nretry = (int64_t)c.sc_host->nretry;
+            r->nstaged = (int64_t)c.sc_host->nstaged;
             return FIVO_STEP_FAILED;
         }
         tau += used;
@@ -880,6 +891,8 @@ int fivo_run(fivo_ctx* ctx, const fivo_run_opts* o, fivo_run_result* r) {
     r->nlearned = (int64_t)c.sc_host->nlearned;
     r->nlearned_fallback = (int64_t)c.sc_host->nfallback;
     r->nqueued = (int64_t)c.sc_host->nqueued;
+    r->nretry = (int64_t)c.sc_host->nretry;
+    r->nstaged = (int64_t)c.sc_host->nstaged;
     return FIVO_OK;
 }
 

@@ -366,6 +366,8 @@ int fivo_batch_run(fivo_batch* bp, const fivo_run_opts* o, const double* tau0, f
         r.nlearned = (int64_t)c.sc_host->nlearned;
         r.nlearned_fallback = (int64_t)c.sc_host->nfallback;
         r.nqueued = (int64_t)c.sc_host->nqueued;
+        r.nretry = (int64_t)c.sc_host->nretry;
+        r.nstaged = (int64_t)c.sc_host->nstaged;
         if (k.state == EV_FAILED) {
             r.ok = 0;
             r.maxu = r.dQ = r.Q1 = r.minPtot = nan;

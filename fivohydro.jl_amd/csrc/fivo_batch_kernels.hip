@@ -85,9 +85,10 @@ __global__ __launch_bounds__(256) void kb_prim(BatchRun R, int phase) {
     const double* U = E->U[(c->cur + phase_in_role(phase)) & 3];
     const double tau = phase_tau(c, phase);
     const int i = (blockIdx.x - E->b0[KB_CELLS]) * 256 + threadIdx.x;
-    unsigned long long its = 0, nfail = 0, nvac = 0, nq = 0;
-    if (i < p.Ntot) prim_cell<EK, NCH>(U, p, tau, w, i, e, &E->sc->ncold, E->jobs, its, nfail, nvac, nq);
+    unsigned long long its = 0, nfail = 0, nvac = 0, nq = 0, nre = 0;
+    if (i < p.Ntot) prim_cell<EK, NCH>(U, p, tau, w, i, e, &E->sc->ncold, E->jobs, its, nfail, nvac, nq, nre);
     prim_counts(E->sc, its, nfail, nvac, nq);
+    rare_counts(E->sc, nre, 0);
 }
 
 // ---- primitive pass: the event's queued cells, G lanes per cell -----------------------------
@@ -112,14 +113,16 @@ __global__ __launch_bounds__(64, G == 8 ? 2 : 1) void kb_coop(BatchRun R, int ph
     const double* U = E->U[(c->cur + phase_in_role(phase)) & 3];
     const double tau = phase_tau(c, phase);
     const PrimJob* jobs = E->jobs;
-    unsigned long long its = 0, nfail = 0, nvac = 0, nfb = 0;
+    unsigned long long its = 0, nfail = 0, nvac = 0, nfb = 0, nre = 0, nst = 0;
     for (; t < n; t += ngroups) {
         int itsum;
-        coop_job<EK, NCH, G>(U, p, tau, w, jobs[t], itsum, nfail, nvac, nfb);
+        coop_job<EK, NCH, G>(U, p, tau, w, jobs[t], itsum, nfail, nvac, nfb, nre, nst);
         if (glane<G>() == 0) its += itsum;
     }
     prim_counts(E->sc, its, nfail, nvac);
     if (glane<G>() == 0 && nfb) atomicAdd(&E->sc->nfallback, nfb);
+    const bool lead = glane<G>() == 0;
+    rare_counts(E->sc, lead ? nre : 0, lead ? nst : 0);
 }
 
 // ---- learned seeds of the event's queued cold cells, 16 per wavefront ---------------------

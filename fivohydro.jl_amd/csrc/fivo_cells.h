@@ -146,6 +146,17 @@ __device__ __forceinline__ void prim_finalize(const Params& p, const DevWork& w,
     }
 }
 
+// the rare recovery branches (fresh-seed retries, staged phi fallbacks): one ballot
+// when none happened in the wavefront
+__device__ __forceinline__ void rare_counts(DevScalars* sc, unsigned long long nre, unsigned long long nst) {
+    if (__ballot(nre != 0 || nst != 0) == 0) return;
+    nre = wave_sum(nre); nst = wave_sum(nst);
+    if ((threadIdx.x & 63) == 0) {
+        if (nre) atomicAdd(&sc->nretry, nre);
+        if (nst) atomicAdd(&sc->nstaged, nst);
+    }
+}
+
 __device__ __forceinline__ void prim_counts(DevScalars* sc, unsigned long long its, unsigned long long nfail,
                                             unsigned long long nvac, unsigned long long nq = 0) {
     its = wave_sum(its); nfail = wave_sum(nfail); nvac = wave_sum(nvac); nq = wave_sum(nq);
@@ -162,7 +173,8 @@ template <int EK, bool NCH>
 __device__ __forceinline__ void prim_cell(const double* __restrict__ U, const Params& p, double tau,
                                           const DevWork& w, int i, int ev, unsigned long long* njobs,
                                           PrimJob* jobs, unsigned long long& its, unsigned long long& nfail,
-                                          unsigned long long& nvac, unsigned long long& nq) {
+                                          unsigned long long& nvac, unsigned long long& nq,
+                                          unsigned long long& nre) {
     const double D = safe_div(U[i], tau), Sx = U[P_ + i], Sy = U[2 * P_ + i], E = U[3 * P_ + i];
     const CellIn c = make_cell<NCH>(D, Sx, Sy, E, load_diss(U, p, i));
     const double yT0 = w.x0_yT[i], ux0 = w.x0_ux[i], uy0 = w.x0_uy[i];
@@ -192,6 +204,7 @@ __device__ __forceinline__ void prim_cell(const double* __restrict__ U, const Pa
         if (!paused) {
             its = r.iters;
             if (r.ok || r.reason == PRR_VACUUM) { prim_finalize(p, w, i, r, c.d, nfail, nvac); job = -1; }
+            else if (warm) nre += 1;             // the warm start failed: the cell retries from the cold seed
         }
     }
     if (job >= 0) {
@@ -213,7 +226,8 @@ constexpr int LEARNED_ITCAP = 12;
 template <int EK, bool NCH, int G>
 __device__ __forceinline__ void coop_job(const double* __restrict__ U, const Params& p, double tau, const DevWork& w,
                                          const PrimJob& jb, int& itsum, unsigned long long& nfail,
-                                         unsigned long long& nvac, unsigned long long& nfallback) {
+                                         unsigned long long& nvac, unsigned long long& nfallback,
+                                         unsigned long long& nre, unsigned long long& nst) {
     const int i = jb.cell;
     const double D = safe_div(U[i], tau), Sx = U[P_ + i], Sy = U[2 * P_ + i], E = U[3 * P_ + i];
     const CellIn c = make_cell<NCH>(D, Sx, Sy, E, load_diss(U, p, i));
@@ -246,8 +260,9 @@ __device__ __forceinline__ void coop_job(const double* __restrict__ U, const Par
             r = newton_finish<EK, NCH, G>(s, c, p.eos);
         }
         itsum += r.iters;
+        nst += r.staged;
         if (mode == 0 || r.ok || r.reason == PRR_VACUUM) break;
-        if (mode == 2) nfallback += 1;
+        if (mode == 2) nfallback += 1; else nre += 1;   // failed learned start / failed warm start
         mode = 0;
     }
     if (glane<G>() == 0) prim_finalize(p, w, i, r, c.d, nfail, nvac);

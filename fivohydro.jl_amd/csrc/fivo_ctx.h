@@ -56,6 +56,8 @@ struct DevScalars {
     unsigned long long nqueued;     // cumulative: cells queued for the cooperative kernel
     unsigned long long bg_umax_bits;// background slice: max |u|, max |v| (double bits; NaN sticks)
     unsigned long long bg_vmax_bits;
+    unsigned long long nretry;      // cumulative: warm-started solves that failed and were retried from the cold seed
+    unsigned long long nstaged;     // cumulative: solves closed by the staged phi fallback (bisection of the charge row)
 };
 
 // Per-cell work arrays on the device (Work2D, src2d/work2d.jl:15-100)

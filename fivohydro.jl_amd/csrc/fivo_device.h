@@ -292,7 +292,7 @@ __device__ __forceinline__ double nu_tau(double ux, double uy, double ut, double
 struct Diss { double nux, nuy, Pi, pixx, pixy, piyy, pieta; };
 
 // ---- primitive recovery (src2d/primrec2d.jl:300-742) --------------------------
-struct PrimRes { double T, mu, ux, uy, n, e, P; int ok, reason, iters; };
+struct PrimRes { double T, mu, ux, uy, n, e, P; int ok, reason, iters; int staged = 0; };   // staged: phi by bisection
 
 // Group-cooperative form: G consecutive lanes of a wavefront solve one cell
 // (G = 8 or 16; G = 0 is the one-lane form). Every lane of a group holds the
@@ -307,17 +307,14 @@ template <int G> __device__ __forceinline__ unsigned gbits(bool pred) {   // pre
     const unsigned long long b = __ballot(pred);
     return (unsigned)((b >> ((threadIdx.x & 63) & ~(G - 1))) & ((1ull << G) - 1));
 }
-__device__ __forceinline__ double sel8(int k, double a0, double a1, double a2, double a3, double a4, double a5,
-                                       double a6) {
-    double v = a6;
-    v = k == 5 ? a5 : v; v = k == 4 ? a4 : v; v = k == 3 ? a3 : v;
-    v = k == 2 ? a2 : v; v = k == 1 ? a1 : v; v = k == 0 ? a0 : v;
-    return v;
-}
 
 // `nsteps` steps of  mid = (lo+hi)/2; pred(mid) ? hi = mid : lo = mid.
-// COOP: three steps per round; lanes 0..6 take the 7 midpoints of the 3-level
-// tree below [lo, hi] (node n has children 2n+1 for pred true, 2n+2 otherwise).
+// COOP: L = log2(G) steps per round (3, 4 or 5 for 8, 16 or 32 lanes); lane l
+// takes the midpoint of node l of the L-level tree below [lo, hi] (node n has
+// children 2n+1 for pred true, 2n+2 otherwise), reached by the same halvings the
+// sequential loop would make on its way there, so every midpoint is bit for bit
+// the one the sequential bisection evaluates; the predicates' ballot then replays
+// the sequential choices.
 template <int G, class Pred>
 __device__ __forceinline__ void bisect(double& lo, double& hi, int nsteps, Pred pred) {
     if constexpr (G == 0) {
@@ -327,19 +324,27 @@ __device__ __forceinline__ void bisect(double& lo, double& hi, int nsteps, Pred 
         }
         return;
     }
+    constexpr int LG = G >= 32 ? 5 : (G >= 16 ? 4 : 3);
     const int l = glane<G>();
     for (int done = 0; done < nsteps;) {
-        const int L = nsteps - done < 3 ? nsteps - done : 3;
-        const double m0 = 0.5 * (lo + hi);
-        const double m1 = 0.5 * (lo + m0), m2 = 0.5 * (m0 + hi);
-        const double m3 = 0.5 * (lo + m1), m4 = 0.5 * (m1 + m0), m5 = 0.5 * (m0 + m2), m6 = 0.5 * (m2 + hi);
+        const int L = nsteps - done < LG ? nsteps - done : LG;
         const int nn = (1 << L) - 1;
         bool pl = false;
-        if (l < nn) pl = pred(sel8(l, m0, m1, m2, m3, m4, m5, m6));
+        if (l < nn) {
+            // descend to node l: the bits of l + 1 below its leading one, 0 = pred true
+            const int m = l + 1;
+            const int d = 31 - __builtin_clz((unsigned)m);
+            double a = lo, z = hi;
+            for (int j = d - 1; j >= 0; --j) {
+                const double mid = 0.5 * (a + z);
+                if ((m >> j) & 1) a = mid; else z = mid;
+            }
+            pl = pred(0.5 * (a + z));
+        }
         const unsigned b = gbits<G>(pl);
         int node = 0;
         for (int lev = 0; lev < L; ++lev) {
-            const double mid = sel8(node, m0, m1, m2, m3, m4, m5, m6);
+            const double mid = 0.5 * (lo + hi);
             const bool p = (b >> node) & 1u;
             if (p) hi = mid; else lo = mid;
             node = 2 * node + (p ? 1 : 2);
@@ -494,6 +499,28 @@ __device__ __forceinline__ bool solve_phi_bisect(double D, double ux, double uy,
     return isf(nm);
 }
 
+// A lane's memo of EOS T-parts keyed by the exact log-temperature they were taken
+// at: the same yT gives the same T and T-part, so a hit changes no result (the
+// reference's memo, EOSCache2D in src2d/primrec2d.jl:115-206, is bit-neutral the
+// same way). A cooperative group's lanes evaluate the finite-difference columns and
+// line-search trials at points whose yT repeats from one iteration to the next once
+// T and u have converged: in a charge-free cell's phi walk (the cold retry, ~56
+// iterations) yT stays on one double or moves between two neighbouring ones, so
+// two slots let those iterations skip every exp and T-part. Groups of 16 and 32
+// lanes keep it (their registers past 256 live in AGPRs); 8-lane groups, which
+// run two wavefronts per SIMD, do not (it would spill to scratch).
+struct TCache {
+    double y[2], T[2];
+    TPart tp[2];
+};
+__device__ __forceinline__ int tcache_slot(const TCache& c, double y0) {
+    return y0 == c.y[0] ? 0 : (y0 == c.y[1] ? 1 : -1);
+}
+__device__ __forceinline__ void tcache_put(TCache& c, double y0, double T, const TPart& tp) {
+    c.y[1] = c.y[0]; c.T[1] = c.T[0]; c.tp[1] = c.tp[0];
+    c.y[0] = y0; c.T[0] = T; c.tp[0] = tp;
+}
+
 // The Newton iterate of _cons_to_prim_attempt_2d!. `it` is the reference's loop
 // variable: an iterate can stop before iteration `it` and resume from (x, it),
 // since T, the T-part and F are pure functions of x.
@@ -509,6 +536,7 @@ struct NewtonState {
     double jcol[4];
     bool jnext, jbad;
     int jbase;      // lane of column 0
+    TCache tc;      // cooperative forms: this lane's T-parts by yT
 };
 constexpr int NEWTON_MAXIT = 80;
 __device__ __forceinline__ bool newton_paused(const NewtonState& s) {
@@ -531,6 +559,7 @@ template <int EK, bool NCH>
 __device__ __forceinline__ bool newton_set(NewtonState& s, const CellIn& c, const EosP& q) {
     s.Tr = exp(s.x[0]);
     s.tp = eos_tpart<EK>(jmax(s.Tr, T_MIN), q);
+    s.tc.y[0] = s.tc.y[1] = __longlong_as_double(0x7ff8000000000000LL);   // empty: NaN matches nothing
     return evalF<EK, NCH>(s.F, s.x[0], s.x[1], s.x[2], s.x[3], s.Tr, s.tp, c, q);
 }
 
@@ -596,7 +625,7 @@ __device__ __forceinline__ void fd_column(const double Fl[4], double h, const Ce
 }
 
 template <int EK, bool NCH, int G>
-__device__ __forceinline__ bool newton_jacobian(const NewtonState& s, const CellIn& c, const EosP& q,
+__device__ __forceinline__ bool newton_jacobian(NewtonState& s, const CellIn& c, const EosP& q,
                                                 double J[4][4]) {
     constexpr int NU = NCH ? 4 : 3;
     if constexpr (G == 0) {
@@ -636,7 +665,17 @@ __device__ __forceinline__ bool newton_jacobian(const NewtonState& s, const Cell
     fd_point<NCH>(s.x, glane<G>(), y, h, act, si);
     double Tl = s.Tr;
     TPart tl = s.tp;
-    if (act && si == 0) { Tl = exp(y[0]); tl = eos_tpart<EK>(jmax(Tl, T_MIN), q); }
+    if (act && si == 0) {
+        const int slot = G >= 16 ? tcache_slot(s.tc, y[0]) : -1;
+        if (slot >= 0) {
+            Tl = slot == 0 ? s.tc.T[0] : s.tc.T[1];
+            tl = slot == 0 ? s.tc.tp[0] : s.tc.tp[1];
+        } else {
+            Tl = exp(y[0]);
+            tl = eos_tpart<EK>(jmax(Tl, T_MIN), q);
+            if (G >= 16) tcache_put(s.tc, y[0], Tl, tl);
+        }
+    }
     double Fl[4] = {0.0, 0.0, 0.0, 0.0};
     bool okl = true;
     if (act) okl = evalF<EK, NCH>(Fl, y[0], y[1], y[2], y[3], Tl, tl, c, q);
@@ -713,13 +752,19 @@ __device__ __forceinline__ bool newton_linesearch(NewtonState& s, const CellIn& 
         int si = 1;
         if (ahead) fd_point<NCH>(t, (l - 8) & 7, y, h, fdact, si);
         const bool fd0 = fdact && si == 0;
-        const double Tl = exp(y[0]);
+        const int slot = G >= 16 ? tcache_slot(s.tc, y[0]) : -1;
+        const double Tl = slot == 0 ? s.tc.T[0] : (slot == 1 ? s.tc.T[1] : exp(y[0]));
         const double Tn = jmax(Tl, T_MIN);
         TPart tl = s.tp;
         double Fl[4] = {0.0, 0.0, 0.0, 0.0};
         bool accl = false, okl = true;
         if (act || fdact) {
-            if (fd0 || Tn != s.tp.T) tl = eos_tpart<EK>(Tn, q);
+            if (slot >= 0) {
+                if (fd0 || Tn != s.tp.T) tl = slot == 0 ? s.tc.tp[0] : s.tc.tp[1];
+            } else {
+                if (fd0 || Tn != s.tp.T) tl = eos_tpart<EK>(Tn, q);
+                if (G >= 16) tcache_put(s.tc, y[0], Tl, tl);    // tl is the T-part at Tn either way
+            }
             okl = evalF<EK, NCH>(Fl, y[0], y[1], y[2], y[3], Tl, tl, c, q);
             if (act && okl) {
                 double trial = 0.0;
@@ -829,6 +874,7 @@ __device__ __forceinline__ PrimRes newton_finish(NewtonState& s, const CellIn& c
             mu = q.m + T * ph;
             eos_assemble<EK>(tf, mu, q, &P, &n, &e);
             R = {T, mu, ux, uy, jmax(n, 0.0), jmax(e, 0.0), P, 1, okb ? PRR_CONVERGED : PRR_RESIDUAL_TOO_LARGE, iters};
+            R.staged = 1;
             return R;
         }
         R = {T, mu, ux, uy, n, jmax(e, 0.0), P, 0, PRR_RESIDUAL_TOO_LARGE, iters};

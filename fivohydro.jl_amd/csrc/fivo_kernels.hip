@@ -18,9 +18,10 @@ template <int EK, bool NCH>
 __global__ __launch_bounds__(256) void k_prim(const double* __restrict__ U, Params p, double tau, DevWork w,
                                               DevScalars* sc, PrimJob* jobs) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    unsigned long long its = 0, nfail = 0, nvac = 0, nq = 0;
-    if (i < p.Ntot) prim_cell<EK, NCH>(U, p, tau, w, i, 0, &sc->ncold, jobs, its, nfail, nvac, nq);
+    unsigned long long its = 0, nfail = 0, nvac = 0, nq = 0, nre = 0;
+    if (i < p.Ntot) prim_cell<EK, NCH>(U, p, tau, w, i, 0, &sc->ncold, jobs, its, nfail, nvac, nq, nre);
     prim_counts(sc, its, nfail, nvac, nq);
+    rare_counts(sc, nre, 0);
 }
 
 template <int EK, bool NCH, int G>
@@ -28,14 +29,16 @@ __global__ __launch_bounds__(256) void k_prim_coop(const double* __restrict__ U,
                                                    DevScalars* sc, const PrimJob* __restrict__ jobs) {
     const int n = (int)sc->ncold;
     const int ngroups = gridDim.x * (blockDim.x / G);
-    unsigned long long its = 0, nfail = 0, nvac = 0, nfb = 0;
+    unsigned long long its = 0, nfail = 0, nvac = 0, nfb = 0, nre = 0, nst = 0;
     for (int t = (blockIdx.x * blockDim.x + threadIdx.x) / G; t < n; t += ngroups) {
         int itsum;
-        coop_job<EK, NCH, G>(U, p, tau, w, jobs[t], itsum, nfail, nvac, nfb);
+        coop_job<EK, NCH, G>(U, p, tau, w, jobs[t], itsum, nfail, nvac, nfb, nre, nst);
         if (glane<G>() == 0) its += itsum;
     }
     prim_counts(sc, its, nfail, nvac);
     if (glane<G>() == 0 && nfb) atomicAdd(&sc->nfallback, nfb);
+    const bool lead = glane<G>() == 0;
+    rare_counts(sc, lead ? nre : 0, lead ? nst : 0);
 }
 
 // learned seeds for the queued cold cells, one wavefront per 16 cells

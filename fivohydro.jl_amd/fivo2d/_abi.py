@@ -80,7 +80,8 @@ WORK_FIELDS = ("yT", "phi", "mu", "alpha", "ux", "uy", "P", "n", "e", "ok", "x0_
 
 
 class Counts(C.Structure):
-    _fields_ = [("nprimfail", C.c_int64), ("nvacuum", C.c_int64), ("amax", C.c_double)]
+    _fields_ = [("nprimfail", C.c_int64), ("nvacuum", C.c_int64), ("amax", C.c_double),
+                ("nretry", C.c_int64), ("nstaged", C.c_int64)]
 
 
 DUMP_CB = C.CFUNCTYPE(None, C.c_void_p, C.c_double, C.c_void_p)
@@ -102,7 +103,7 @@ class RunResult(C.Structure):
                 ("Q0", C.c_double), ("Q1", C.c_double), ("minPtot", C.c_double),
                 ("nrhs", C.c_int64), ("nhalvings", C.c_int64), ("nmood", C.c_int64),
                 ("newton_iters", C.c_int64), ("nlearned", C.c_int64), ("nlearned_fallback", C.c_int64),
-                ("nqueued", C.c_int64)]
+                ("nqueued", C.c_int64), ("nretry", C.c_int64), ("nstaged", C.c_int64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_ if not f.startswith("_")}

@@ -122,10 +122,14 @@ typedef struct fivo_work_view {
     double *shear_res;
 } fivo_work_view;
 
-/* Per-call counters of the primitive pass (rhs2d.jl:89-112). */
+/* Per-call counters of the primitive pass (rhs2d.jl:89-112). nretry and nstaged,
+ * not in the reference: cells whose warm-started solve failed and were retried
+ * from the cold seed (cons_to_prim_2d!, src2d/primrec2d.jl:465-486), and solves
+ * closed by the staged phi fallback (src2d/primrec2d.jl:723-738). */
 typedef struct fivo_counts {
     int64_t nprimfail, nvacuum;
     double amax;
+    int64_t nretry, nstaged;
 } fivo_counts;
 
 /* run_sim_2d! options (main2D.jl:371-378) */
@@ -155,6 +159,8 @@ typedef struct fivo_run_result {
     int64_t nlearned;            /* cells started from the learned seed */
     int64_t nlearned_fallback;   /* of those, cells that fell back to the cold seed */
     int64_t nqueued;             /* cells handed from the warm to the cooperative primitive kernel */
+    int64_t nretry;              /* warm-started solves retried from the cold seed (primrec2d.jl:465-486) */
+    int64_t nstaged;             /* solves closed by the staged phi fallback (primrec2d.jl:723-738) */
 } fivo_run_result;
 
 typedef struct fivo_ctx fivo_ctx;

@@ -169,14 +169,20 @@ int orc_apply_bc(Ctx* c, int bc) { apply_bc(*c, c->U, bc); return FIVO_OK; }
 int orc_rhs(Ctx* c, double tau, int bc, const uint8_t* force, double* dU, fivo_counts* cnt) {
     rhs(*c, c->w.k, c->U, tau, bc, force);
     if (dU) std::memcpy(dU, c->w.k.data(), sizeof(double) * c->w.k.size());
-    if (cnt) { cnt->nprimfail = c->w.primfail; cnt->nvacuum = c->w.vacuum; cnt->amax = c->w.amax; }
+    if (cnt) {
+        cnt->nprimfail = c->w.primfail; cnt->nvacuum = c->w.vacuum; cnt->amax = c->w.amax;
+        cnt->nretry = c->w.nretry; cnt->nstaged = c->w.nstaged;
+    }
     return FIVO_OK;
 }
 // update_primitives_2d! (src2d/rhs2d.jl:57-65)
 int orc_update_primitives(Ctx* c, double tau, int bc, fivo_counts* cnt) {
     apply_bc(*c, c->U, bc);
     primitive_pass(*c, c->U, tau);
-    if (cnt) { cnt->nprimfail = c->w.primfail; cnt->nvacuum = c->w.vacuum; cnt->amax = c->w.amax; }
+    if (cnt) {
+        cnt->nprimfail = c->w.primfail; cnt->nvacuum = c->w.vacuum; cnt->amax = c->w.amax;
+        cnt->nretry = c->w.nretry; cnt->nstaged = c->w.nstaged;
+    }
     return FIVO_OK;
 }
 // export_background2d's slice loop (tools/export_background2d.jl:79-94): T = exp(yT),
@@ -231,7 +237,7 @@ int orc_run(Ctx* c, const fivo_run_opts* o, fivo_run_result* res) {
     const int nv = L.nv, ng = g.ng;
     const fivo_model_desc& d = c->m.d;
     std::memset(res, 0, sizeof(*res));
-    c->nrhs = 0; c->nhalvings = 0; c->nmood = 0; w.newton_iters = 0;
+    c->nrhs = 0; c->nhalvings = 0; c->nmood = 0; w.newton_iters = 0; w.nretry_run = 0; w.nstaged_run = 0;
     if (o->reset_history) reset_history(*c);
     rhs(*c, w.k, c->U, o->tau0, o->bc, nullptr);
     double Qin = 0.0;
@@ -260,6 +266,7 @@ int orc_run(Ctx* c, const fivo_run_opts* o, fivo_run_result* res) {
             res->maxu = res->dQ = res->Q1 = res->minPtot = NaNv; res->Q0 = Qin;
             res->nrhs = c->nrhs; res->nhalvings = c->nhalvings; res->nmood = c->nmood;
             res->newton_iters = w.newton_iters;
+            res->nretry = w.nretry_run; res->nstaged = w.nstaged_run;
             return FIVO_STEP_FAILED;
         }
         tau += used;
@@ -297,6 +304,7 @@ int orc_run(Ctx* c, const fivo_run_opts* o, fivo_run_result* res) {
     res->Q0 = Qin; res->Q1 = Qout; res->minPtot = minPtot;
     res->nrhs = c->nrhs; res->nhalvings = c->nhalvings; res->nmood = c->nmood;
     res->newton_iters = w.newton_iters;
+    res->nretry = w.nretry_run; res->nstaged = w.nstaged_run;
     return FIVO_OK;
 }
 

@@ -106,7 +106,7 @@ static inline void evalF(double* F, const double* x, const Cons& c, const Diss& 
 }
 
 struct PrimOut { double T, mu, ux, uy, n, e, P; bool ok; };
-struct PrimStat { uint8_t reason; int iters; double resnorm; double rows[4]; };
+struct PrimStat { uint8_t reason; int iters; double resnorm; double rows[4]; int staged = 0; };   // staged: phi by bisection
 
 // _solve_phi_bisect (src2d/primrec2d.jl:416-434)
 static inline bool solve_phi_bisect(double D, double T, double ux, double uy, double nux, double nuy,
@@ -245,6 +245,7 @@ static inline PrimOut cons_to_prim_attempt(const Cons& c, const Diss& d, double 
             mu = q.mass() + T * x[1];
             r = eos_Pne_2d(T, mu, q, &ec);
             st.reason = okb ? PRR_CONVERGED : PRR_RESIDUAL_TOO_LARGE;
+            st.staged = 1;
             return {T, mu, ux, uy, jmax(r.n, 0.0), jmax(r.e, 0.0), r.P, true};
         }
         st.reason = PRR_RESIDUAL_TOO_LARGE;

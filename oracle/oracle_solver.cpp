@@ -63,8 +63,8 @@ void primitive_pass(Ctx& c, std::vector<double>& U, double tau) {
     const EOS& q = c.m.eos;
     Work& w = c.w;
     const int nv = L.nv;
-    int64_t nfail = 0, nvac = 0, nit = 0;
-#pragma omp parallel for schedule(dynamic, 1) reduction(+ : nfail, nvac, nit)
+    int64_t nfail = 0, nvac = 0, nit = 0, nre = 0, nst = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : nfail, nvac, nit, nre, nst)
     for (int ix = 0; ix < g.Nxtot; ++ix) {
         for (int iy = 0; iy < g.Nytot; ++iy) {
             int i = g.lin(ix, iy);
@@ -77,6 +77,8 @@ void primitive_pass(Ctx& c, std::vector<double>& U, double tau) {
             PrimOut r = cons_to_prim(cs, d, tau, q, w.x0_yT[i], w.x0_phi[i], w.x0_ux[i], w.x0_uy[i],
                                      st, &its, &it1, &natt);
             nit += its;
+            nre += natt == 2 ? 1 : 0;
+            nst += st.staged;
             if (!w.dbg_iters.empty()) {
                 w.dbg_iters[i] = (st.reason == PRR_VACUUM) ? -1 : its;
                 // class: 1 warm start, 2 fresh-seed retry, 4 ok; first attempt's iterations above bit 8
@@ -110,6 +112,7 @@ void primitive_pass(Ctx& c, std::vector<double>& U, double tau) {
         }
     }
     w.primfail = nfail; w.vacuum = nvac; w.newton_iters += nit;
+    w.nretry = nre; w.nstaged = nst; w.nretry_run += nre; w.nstaged_run += nst;
     if (!w.dbg_iters.empty()) {
         if (w.dbg_hist.size() < 256) w.dbg_hist.assign(256, 0);
         if (w.dbg_hist2.size() < 2 * 8 * 256) w.dbg_hist2.assign(2 * 8 * 256, 0);

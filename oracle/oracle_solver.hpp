@@ -73,6 +73,8 @@ struct Work {
     std::vector<int64_t> dbg_hist2;  // [total|first-attempt][class][iterations]
     std::vector<int> dbg_class;      // per cell: warm start, retry, ok, first-attempt iterations
     int64_t primfail = 0, vacuum = 0, newton_iters = 0;
+    int64_t nretry = 0, nstaged = 0;          // last primitive pass: fresh-seed retries, staged phi fallbacks
+    int64_t nretry_run = 0, nstaged_run = 0;  // the same, summed since the run started
     double amax = 0.0;
 };
 

@@ -211,38 +211,61 @@ def test_full_length_run_vs_oracle(oracle, hip, name, tauf):
     assert dT <= 1e-9 and du <= 1e-9
 
 
-def test_full_length_c3_conditioning(hip):
-    """Configuration 3 (512^2 full sectors, tau 0.4 -> 10, SSPRK3) on its lumpy event is
-    ill-conditioned at full length: the same run with the initial conserved rows
-    perturbed by 1e-13 (relative) departs by far more than 1e-9, with recovery-flag
-    flips. So final-time agreement to 1e-9 with the reference on this configuration
-    is out of reach of any implementation that is not bit-identical to it; the HIP
-    path's measured distance from the oracle at full length (tests/run_full_length.py:
-    2.9e-7, no flips, the same 802 steps) is far inside this spread."""
+def test_full_length_c3_vs_oracle_and_conditioning(hip, oracle):
+    """Configuration 3 (512^2 full sectors, tau 0.4 -> 10, SSPRK3) to its final time
+    on the HIP path and on the oracle from the same input, and on the HIP path again
+    with the initial conserved rows perturbed by 1e-13 (relative). HIP and oracle take
+    the same 802 steps with the same counters and no recovery-flag flips; their final
+    T differs by ~3e-7 (measured 2.9e-7; 5.8e-7 with multiply-add fusion off on the
+    device, so the device's contraction is not its source: the two sides' exp and log
+    round differently in the last bit). That distance is the reference algorithm's
+    conditioning on this lumpy event, not a discrepancy: the 1e-13 perturbation alone
+    moves the final T by ~7e-2 with recovery-flag flips, so no implementation that is
+    not bit-identical to the reference can agree with it to 1e-9 at this length. The
+    test asserts the HIP-vs-oracle distance stays below 1e-5 and below 1e-3 of the
+    perturbation's."""
+    import ctypes as C
     import sys
+    import time
     from conftest import REPO
     if REPO not in sys.path:
         sys.path.insert(0, REPO)
     import bench
     cfg = bench.CONFIGS["c3"]
-    Ts, oks = [], []
-    for eps in (0.0, 1e-13):
+    oracle.dll.orc_set_threads.restype = C.c_int
+    oracle.dll.orc_set_threads.argtypes = [C.c_int]
+    oracle.dll.orc_set_threads(16)
+    runs = {}
+    for tag, lib, eps in (("hip", hip, 0.0), ("hip_perturbed", hip, 1e-13), ("oracle", oracle, 0.0)):
         g, m, prims = bench.make_inputs(cfg)
-        s = F.Solver(g, m, lib=hip)
+        s = F.Solver(g, m, lib=lib)
         s.init_from_prims(cfg["tau0"], **prims)
         if eps:
             U = s.get_state()
             U[:, :4] *= 1 + eps * np.random.default_rng(1).uniform(-1, 1, size=(U.shape[0], 4))
             s.set_state(U)
+        t0 = time.perf_counter()
         r = s.run_sim_2d(tau0=cfg["tau0"], taufinal=10.0, CFL=cfg["cfl"], CFLtau=cfg["cfltau"],
-                         integrator=cfg["integrator"])
+                         integrator=cfg["integrator"], dump_dt=2.0,
+                         on_dump=lambda tau, _s, tag=tag: print(f"  {tag} tau {tau:.2f}", flush=True))
         assert r["ok"]
         w = s.get_work(("yT", "ok"))
-        Ts.append(np.exp(w["yT"]))
-        oks.append(w["ok"])
+        runs[tag] = (r, np.exp(w["yT"]), w["ok"])
+        print(f"{tag}: {r['nsteps']} steps in {time.perf_counter() - t0:.1f} s", flush=True)
         s.close()
-    both = (oks[0] & oks[1]).astype(bool)
-    d = np.max(np.abs(Ts[1] - Ts[0])[both] / Ts[0][both])
-    flips = int(np.sum(oks[0] != oks[1]))
-    print(f"c3 to tau 10, initial state perturbed by 1e-13: max dT/T {d:.2e}, recovery-flag flips {flips}", flush=True)
-    assert d > 1e-6
+
+    def dist(a, b):
+        (_, Ta, oka), (_, Tb, okb) = runs[a], runs[b]
+        both = (oka == 1) & (okb == 1)
+        return float(np.max(np.abs(Ta - Tb)[both] / Tb[both])), int(np.sum(oka != okb))
+
+    (rh, _, _), (ro, _, _) = runs["hip"], runs["oracle"]
+    for k in ("nsteps", "nhalvings", "nmood", "nprimfail", "tau"):
+        assert rh[k] == ro[k], k
+    d_ho, f_ho = dist("hip", "oracle")
+    d_pp, f_pp = dist("hip_perturbed", "hip")
+    print(f"c3 to tau 10: HIP vs oracle max dT/T {d_ho:.2e} ({f_ho} flips); HIP with the input perturbed by "
+          f"1e-13 vs HIP: {d_pp:.2e} ({f_pp} flips)", flush=True)
+    assert f_ho == 0
+    assert d_pp > 1e-6
+    assert d_ho < 1e-5 and d_ho < 1e-3 * d_pp
