@@ -411,8 +411,10 @@ int fivo_create(const fivo_grid_desc* g, const fivo_model_desc* m, fivo_ctx** ou
         return fail(nullptr, FIVO_ERR_ARG, "grid: need nx, ny > 8, nghost = 3, xmax, ymax > 0");
     if (m->eos.kind != FIVO_EOS_CONFORMAL_HQ && m->eos.kind != FIVO_EOS_LATTICE_HRG)
         return fail(nullptr, FIVO_ERR_ARG, "unknown EOS kind");
-    if (m->advect_nu || m->advect_Pi || m->advect_pi)
-        return fail(nullptr, FIVO_ERR_ARG, "advect_* (flux transport of dissipatives) is not supported");
+    // build_model_2d (main2D.jl:85-123): one transport per dissipative field
+    if ((m->advect_nu && m->relax_advect_nu) || (m->advect_Pi && m->relax_advect_Pi) ||
+        (m->advect_pi && m->relax_advect_pi))
+        return fail(nullptr, FIVO_ERR_ARG, "Pick ONE for transport: advect_* OR relax_advect_* (not both)");
     if (m->consistent_fm && m->tauN_coeff != 1.0)
         return fail(nullptr, FIVO_ERR_ARG, "tauN_coeff cannot be combined with consistent_fm = true");
     if (m->consistent_m2 && !m->enable_diff)
@@ -471,6 +473,8 @@ int fivo_create(const fivo_grid_desc* g, const fivo_model_desc* m, fivo_ctx** ou
     p.consistent_fm = m->consistent_fm; p.consistent_m2 = m->consistent_m2;
     p.relax_advect_pi = m->relax_advect_pi; p.relax_advect_Pi = m->relax_advect_Pi;
     p.relax_advect_nu = m->relax_advect_nu; p.relax_advect_m2 = m->relax_advect_m2;
+    p.advect_nu = p.hasNu && m->advect_nu; p.advect_Pi = p.hasPi && m->advect_Pi;
+    p.advect_pi = p.hasShear && m->advect_pi;
     p.shear_projected_deriv = m->shear_projected_deriv; p.diff_projected_deriv = m->diff_projected_deriv;
     p.vacuum_ramp_relax = m->vacuum_ramp_relax; p.terms = m->terms;
     p.eta_over_s = m->eta_over_s; p.Cs = m->tauShear_coeff; p.deltaShear_factor = m->deltaShear_factor;

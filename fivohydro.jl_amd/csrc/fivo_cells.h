@@ -382,7 +382,8 @@ __device__ __forceinline__ bool p2c(double T, double mu, double ux, double uy, c
     return true;
 }
 
-// flux_cell_2d! (src2d/fluxes2d.jl:24-69), conserved rows (advect_* are off)
+// flux_cell_2d! (src2d/fluxes2d.jl:24-69), the conserved rows (the advected
+// dissipative rows are formed by the face, below)
 __device__ __forceinline__ void flux4(const FPrim& s, const Diss& d, double tau, int dir, double F[4]) {
     const double ux = s.ux, uy = s.uy;
     const double ut = sqrt(1 + ux * ux + uy * uy);
@@ -544,15 +545,27 @@ __device__ __forceinline__ double face(const FaceK& K, int i, int dir, int k0, b
         for (int a = 0; a < 4; ++a)
             out[a * ostride] = mode == 0 ? FL[a] : (mode == 1 ? FR[a] : (a_r * FL[a] - a_l * FR[a] + a_r * a_l * (cR[a] - cL[a])) * inv);
         if (p.nv > 4) {
-            // dissipative rows: zero physical flux, HLLE jump of the stored values
-            // (zero on a floor side); charm second-moment rows never enter the face states
+            // dissipative rows: physical flux stored * v_n when the field is advected
+            // (advect_*, src2d/fluxes2d.jl:55-67), zero otherwise, and the HLLE jump of
+            // the stored values (zero on a floor side); charm second-moment rows never
+            // enter the face states
             const double jf = mode == 2 ? a_r * a_l : 0.0;
-            auto row = [&](int idx, double l, double r) {
-                if (idx >= 0) out[idx * ostride] = mode == 2 ? (a_r * 0.0 - a_l * 0.0 + jf * (r - l)) * inv : 0.0;
+            double vL = 0.0, vR = 0.0;
+            if (p.advect_nu || p.advect_Pi || p.advect_pi) {
+                const double iuL = safe_inv(sqrt(1 + sL.ux * sL.ux + sL.uy * sL.uy));
+                const double iuR = safe_inv(sqrt(1 + sR.ux * sR.ux + sR.uy * sR.uy));
+                vL = (dir == 0 ? sL.ux : sL.uy) * iuL;
+                vR = (dir == 0 ? sR.ux : sR.uy) * iuR;
+            }
+            auto row = [&](int idx, double l, double r, int adv) {
+                if (idx < 0) return;
+                const double fl = adv ? l * vL : 0.0, fr = adv ? r * vR : 0.0;
+                out[idx * ostride] = mode == 0 ? fl : (mode == 1 ? fr : (a_r * fl - a_l * fr + jf * (r - l)) * inv);
             };
-            row(p.iNux, zL.nux, zR.nux); row(p.iNuy, zL.nuy, zR.nuy); row(p.iPi, zL.Pi, zR.Pi);
-            row(p.iPixx, zL.pixx, zR.pixx); row(p.iPixy, zL.pixy, zR.pixy);
-            row(p.iPiyy, zL.piyy, zR.piyy); row(p.iPieta, zL.pieta, zR.pieta);
+            row(p.iNux, zL.nux, zR.nux, p.advect_nu); row(p.iNuy, zL.nuy, zR.nuy, p.advect_nu);
+            row(p.iPi, zL.Pi, zR.Pi, p.advect_Pi);
+            row(p.iPixx, zL.pixx, zR.pixx, p.advect_pi); row(p.iPixy, zL.pixy, zR.pixy, p.advect_pi);
+            row(p.iPiyy, zL.piyy, zR.piyy, p.advect_pi); row(p.iPieta, zL.pieta, zR.pieta, p.advect_pi);
         }
     }
     const double aa = jmax(fabs(a_l), fabs(a_r));
@@ -716,15 +729,30 @@ __device__ __forceinline__ void faces_body(const FaceK& K, const int bid, double
             };
             // geometric source (src2d/fluxes2d.jl:166-177)
             const double S1 = -b.Sx * itau, S2 = -b.Sy * itau, S3 = -(b.E + (Pw + b.Pi) + b.pieta) * itau;
+            // advected dissipative fields: + q div v, div v from the cell velocities
+            // (compute_divv_2d!, src2d/rhs2d.jl:29-41, 200-224)
+            double divv = 0.0;
+            if (p.advect_nu || p.advect_Pi || p.advect_pi)
+                divv = 0.5 * (K.w.vx[i + p.Nytot] - K.w.vx[i - p.Nytot]) * idx +
+                       0.5 * (K.w.vy[i + 1] - K.w.vy[i - 1]) * idy;
+            const double Snx = p.advect_nu ? b.nux * divv : 0.0, Sny = p.advect_nu ? b.nuy * divv : 0.0;
+            const double SPi = p.advect_Pi ? b.Pi * divv : 0.0;
+            const double Sxx = p.advect_pi ? b.pixx * divv : 0.0, Sxy = p.advect_pi ? b.pixy * divv : 0.0;
+            const double Syy = p.advect_pi ? b.piyy * divv : 0.0, Set = p.advect_pi ? b.pieta * divv : 0.0;
             if (MODE == 0) {
                 double* dU = K.f.dUout;
                 dU[i] = du(0) + 0.0;
                 dU[P_ + i] = du(1) + S1;
                 dU[2 * P_ + i] = du(2) + S2;
                 dU[3 * P_ + i] = du(3) + S3;
-                const int nvf = p.iPQ0 >= 0 ? p.iPQ0 : p.nv;
-                for (int a = 4; a < nvf; ++a) dU[a * P_ + i] = du(a) + 0.0;
-                for (int a = nvf; a < p.nv; ++a) dU[a * P_ + i] = 0.0;
+                if (p.hasNu) { dU[p.iNux * P_ + i] = du(p.iNux) + Snx; dU[p.iNuy * P_ + i] = du(p.iNuy) + Sny; }
+                if (p.hasPi) dU[p.iPi * P_ + i] = du(p.iPi) + SPi;
+                if (p.hasShear) {
+                    dU[p.iPixx * P_ + i] = du(p.iPixx) + Sxx; dU[p.iPixy * P_ + i] = du(p.iPixy) + Sxy;
+                    dU[p.iPiyy * P_ + i] = du(p.iPiyy) + Syy; dU[p.iPieta * P_ + i] = du(p.iPieta) + Set;
+                }
+                if (p.iPQ0 >= 0)
+                    for (int a = p.iPQ0; a < p.nv; ++a) dU[a * P_ + i] = 0.0;
             } else {
                 const double D = K.f.D, wo = K.f.wold, wn = K.f.wnew;
                 const int ct = K.f.combine;
@@ -740,13 +768,13 @@ __device__ __forceinline__ void faces_body(const FaceK& K, const int bid, double
                 s.Sx = comb(1, b.Sx, du(1) + S1);
                 s.Sy = comb(2, b.Sy, du(2) + S2);
                 s.E = comb(3, b.E, du(3) + S3);
-                s.nux = p.hasNu ? comb(p.iNux, b.nux, du(p.iNux) + 0.0) : 0.0;
-                s.nuy = p.hasNu ? comb(p.iNuy, b.nuy, du(p.iNuy) + 0.0) : 0.0;
-                s.Pi = p.hasPi ? comb(p.iPi, b.Pi, du(p.iPi) + 0.0) : 0.0;
-                s.pixx = p.hasShear ? comb(p.iPixx, b.pixx, du(p.iPixx) + 0.0) : 0.0;
-                s.pixy = p.hasShear ? comb(p.iPixy, b.pixy, du(p.iPixy) + 0.0) : 0.0;
-                s.piyy = p.hasShear ? comb(p.iPiyy, b.piyy, du(p.iPiyy) + 0.0) : 0.0;
-                s.pieta = p.hasShear ? comb(p.iPieta, b.pieta, du(p.iPieta) + 0.0) : 0.0;
+                s.nux = p.hasNu ? comb(p.iNux, b.nux, du(p.iNux) + Snx) : 0.0;
+                s.nuy = p.hasNu ? comb(p.iNuy, b.nuy, du(p.iNuy) + Sny) : 0.0;
+                s.Pi = p.hasPi ? comb(p.iPi, b.Pi, du(p.iPi) + SPi) : 0.0;
+                s.pixx = p.hasShear ? comb(p.iPixx, b.pixx, du(p.iPixx) + Sxx) : 0.0;
+                s.pixy = p.hasShear ? comb(p.iPixy, b.pixy, du(p.iPixy) + Sxy) : 0.0;
+                s.piyy = p.hasShear ? comb(p.iPiyy, b.piyy, du(p.iPiyy) + Syy) : 0.0;
+                s.pieta = p.hasShear ? comb(p.iPieta, b.pieta, du(p.iPieta) + Set) : 0.0;
                 if (p.iPQ0 >= 0) {
                     // charm second-moment rows: never in the face states, zero flux
                     for (int k = 0; k < 5; ++k) {

@@ -34,6 +34,9 @@ struct Params {
     // relaxation (src2d/dissipation2d.jl, src/primitives.jl:30-108)
     int enable_shear, enable_bulk, enable_diff, consistent_fm, consistent_m2;
     int relax_advect_pi, relax_advect_Pi, relax_advect_nu, relax_advect_m2;
+    // passive-scalar transport of the dissipative fields through the face flux, with
+    // the grad.v source (advect_*, src2d/fluxes2d.jl:55-67, src2d/rhs2d.jl:200-224)
+    int advect_nu, advect_Pi, advect_pi;
     int shear_projected_deriv, diff_projected_deriv, vacuum_ramp_relax;
     uint32_t terms;
     double eta_over_s, Cs, deltaShear_factor, taupi_pi_factor, pi_clip_factor;

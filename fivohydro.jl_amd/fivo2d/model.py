@@ -363,9 +363,6 @@ def build_model_2d(eos=None, r_domain=math.inf, enable_shear=False, enable_bulk=
                  ("advect_pi", "relax_advect_pi")):
         if p[a] and p[b]:
             raise ValueError(f"Pick ONE for transport: {a} OR {b} (not both).")
-    if p["advect_nu"] or p["advect_Pi"] or p["advect_pi"]:
-        raise ValueError("advect_* (passive-scalar transport through the flux) is not supported by "
-                         "this library; the reference's production default relax_advect_* is.")
     if p["consistent_fm"] and p["tauN_coeff"] != 1.0:
         raise ValueError("`tauN_coeff` cannot be combined with `consistent_fm = true`.")
     if p["consistent_m2"] and not enable_diff:

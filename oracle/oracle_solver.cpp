@@ -344,8 +344,22 @@ void rhs(Ctx& c, std::vector<double>& dU, std::vector<double>& U, double tau, in
     double ax = faces(c, U, tau, 0, E_FLOOR, force);
     double ay = faces(c, U, tau, 1, E_FLOOR, force);
     w.amax = std::max(ax, ay);
-    // sources (src2d/rhs2d.jl:206-231, src2d/fluxes2d.jl:166-177); advect_* paths
-    // (compute_divv_2d!) are refused at create, so theta stays zero here.
+    // sources (src2d/rhs2d.jl:200-231, src2d/fluxes2d.jl:166-177)
+    const fivo_model_desc& md = c.m.d;
+    const bool adv_nu = L.hasNu && md.advect_nu, adv_Pi = L.hasPi && md.advect_Pi,
+               adv_pi = L.hasShear && md.advect_pi;
+    const bool need_div = adv_nu || adv_Pi || adv_pi;
+    std::fill(w.theta.begin(), w.theta.end(), 0.0);
+    if (need_div) {
+        // compute_divv_2d! (src2d/rhs2d.jl:29-41)
+        const double invdx = 1 / g.dx, invdy = 1 / g.dy;
+#pragma omp parallel for schedule(static)
+        for (int ix = 1; ix < g.Nxtot - 1; ++ix)
+            for (int iy = 1; iy < g.Nytot - 1; ++iy) {
+                const int i = g.lin(ix, iy);
+                w.theta[i] = 0.5 * (w.vxC[i + Nyt] - w.vxC[i - Nyt]) * invdx + 0.5 * (w.vyC[i + 1] - w.vyC[i - 1]) * invdy;
+            }
+    }
     std::fill(w.S.begin(), w.S.end(), 0.0);
     const double itau = safe_inv(tau);
 #pragma omp parallel for schedule(static)
@@ -358,6 +372,15 @@ void rhs(Ctx& c, std::vector<double>& dU, std::vector<double>& U, double tau, in
             S[L.iSx] = -Ui[L.iSx] * itau;
             S[L.iSy] = -Ui[L.iSy] * itau;
             S[L.iE] = -(Ui[L.iE] + (w.P[i] + d.Pi) + d.pieta) * itau;
+            if (need_div) {
+                const double divv = w.theta[i];
+                if (adv_nu) { S[L.iNux] += Ui[L.iNux] * divv; S[L.iNuy] += Ui[L.iNuy] * divv; }
+                if (adv_Pi) S[L.iPi] += Ui[L.iPi] * divv;
+                if (adv_pi) {
+                    S[L.iPixx] += Ui[L.iPixx] * divv; S[L.iPixy] += Ui[L.iPixy] * divv;
+                    S[L.iPiyy] += Ui[L.iPiyy] * divv; S[L.iPieta] += Ui[L.iPieta] * divv;
+                }
+            }
         }
     std::fill(dU.begin(), dU.end(), 0.0);
     const double idx = 1 / g.dx, idy = 1 / g.dy;

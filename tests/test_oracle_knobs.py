@@ -13,6 +13,9 @@ oracle's final state within the final-time bound for each.
   shear_projected_deriv  the projected velocity gradient in the NS shear target
   diff_projected_deriv   the projected alpha gradient in the NS charge target
   relax_advect_*         upwind advection inside the relaxation of nu, Pi, pi, the second moment
+  advect_*               passive-scalar transport of nu, Pi, pi through the face flux with the
+                         q div(v) source (fluxes2d.jl:55-67, rhs2d.jl:29-41, 200-224), in place
+                         of relax_advect_* (one transport per field, main2D.jl:85-123)
   deltaN_factor          the delta_N theta term of the charge relaxation
   T_vac_cut              the vacuum cut's temperature (main2D.jl:100-105)
 """
@@ -40,6 +43,9 @@ KNOBS = {
     "relax_advect_m2_off": dict(relax_advect_m2=False),
     "deltaN_factor": dict(deltaN_factor=1.0),
     "T_vac_cut": dict(T_vac_cut=0.09),
+    "advect_nu": dict(advect_nu=True, relax_advect_nu=False),
+    "advect_Pi": dict(advect_Pi=True, relax_advect_Pi=False),
+    "advect_pi": dict(advect_pi=True, relax_advect_pi=False),
 }
 
 
@@ -81,3 +87,34 @@ def test_knob_acts_on_the_oracle(oracle, name):
     top = sorted(moved.items(), key=lambda kv: -kv[1])[:3]
     print(f"{name}: largest changes {', '.join(f'{k} {v:.1e}' for k, v in top)}")
     assert max(d) > 1e-9, name
+
+
+def test_advect_one_transport_per_field():
+    """build_model_2d refuses advect_* together with relax_advect_* for the same field."""
+    for k in ("nu", "Pi", "pi"):
+        with pytest.raises(ValueError, match="Pick ONE"):
+            model(**{f"advect_{k}": True})
+
+
+def test_advected_fields_keep_mirror_symmetry(oracle):
+    """With every dissipative field advected through the flux, an x <-> y symmetric
+    lumpy event stays symmetric row by row (a transposed slot or the wrong velocity
+    component in the advective flux or the div v source breaks it)."""
+    from test_oracle_dissipative_gates import symmetric_event, transpose_rows
+    g = F.make_grid2d(32, 32, xmax=8.0, ymax=8.0)
+    m = model(advect_nu=True, relax_advect_nu=False, advect_Pi=True, relax_advect_Pi=False,
+              advect_pi=True, relax_advect_pi=False)
+    s = F.Solver(g, m, lib=oracle)
+    s.init_from_prims(0.6, **symmetric_event(g))
+    r = s.run_sim_2d(tau0=0.6, taufinal=1.2, integrator="ssprk3")
+    assert r["ok"] and r["nsteps"] > 5
+    U = s.get_state()
+    sx, sy = g.interior()
+    inside = np.zeros((g.Nxtot, g.Nytot), bool)
+    inside[sx, sy] = True
+    Ui = U[inside.ravel()]
+    Vi = transpose_rows(U, g, m.layout)[inside.ravel()]
+    scale = np.maximum(np.max(np.abs(Ui), axis=0), 1e-300)
+    worst = np.max(np.abs(Ui - Vi) / scale)
+    print(f"advected fields: max x<->y asymmetry {worst:.2e}")
+    assert worst < 1e-9
