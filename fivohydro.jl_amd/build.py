@@ -15,7 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(CSRC, "libfivo2d.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 SOURCES = ["fivo_kernels.hip", "fivo_batch_kernels.hip", "fivo_relax.hip", "fivo_api.hip", "fivo_batch.hip",
-           "fivo_mlp.hip"]
+           "fivo_mlp.hip", "fivo_freezeout.hip"]
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-fno-gpu-rdc",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          # multiply-adds fuse only within one source expression, so the same body

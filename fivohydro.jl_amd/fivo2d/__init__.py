@@ -16,8 +16,11 @@ from .mlp import Mlp
 from .background import export_background2d, export_backgrounds2d, load_background
 from .fields_io import save_fields, load_fields
 from . import ic
+from . import freezeout
+from .freezeout import FreezeOut, freeze_surface, pion_yield
 
 __all__ = ["FIVO_BC_OUTFLOW", "FIVO_BC_PERIODIC", "FIVO_SSPRK2", "FIVO_SSPRK3", "Lib", "load_hip",
            "HIP_LIB_PATH", "ConformalHQEOS", "LatticeHRGEOS", "Grid2D", "make_grid2d",
            "make_grid_2d", "make_layout2d", "build_model_2d", "allocate_state", "resolve_terms", "without", "check_terms", "TERM_REGISTER",
-           "Solver", "FivoError", "Ensemble", "Mlp", "export_background2d", "export_backgrounds2d", "load_background", "ic", "T_MIN", "E_FLOOR", "HBARC", "save_fields", "load_fields"]
+           "Solver", "FivoError", "Ensemble", "Mlp", "export_background2d", "export_backgrounds2d", "load_background", "ic", "T_MIN", "E_FLOOR", "HBARC", "save_fields", "load_fields",
+           "freezeout", "FreezeOut", "freeze_surface", "pion_yield"]

@@ -109,6 +109,12 @@ class RunResult(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_ if not f.startswith("_")}
 
 
+class CfOpts(C.Structure):
+    _fields_ = [("n_eta", C.c_int32), ("n_pt", C.c_int32), ("n_phi", C.c_int32), ("_pad", C.c_int32),
+                ("eta_max", C.c_double), ("pt_lo", C.c_double), ("pt_hi", C.c_double),
+                ("mass", C.c_double), ("yield_factor", C.c_double)]
+
+
 # Entry points shared by the HIP library (prefix fivo_) and any library with the
 # same ABI. (name, restype, argtypes)
 _VP = C.c_void_p
@@ -139,6 +145,12 @@ ABI_FUNCS = (
     ("run", C.c_int, [_VP, C.POINTER(RunOpts), C.POINTER(RunResult)]),
     ("probe_fm_source", C.c_int, [C.c_uint32, C.c_int64, _DP, _DP]),
     ("probe_m2_source", C.c_int, [C.c_uint32, C.c_int64, _DP, _DP]),
+    ("freezeout_create", C.c_int, [_VP, C.c_double, C.POINTER(C.c_void_p)]),
+    ("freezeout_destroy", None, [_VP]),
+    ("freezeout_record", C.c_int, [_VP, C.c_double]),
+    ("freezeout_surface", C.c_int, [_VP, _DP, _DP, _DP, C.POINTER(C.c_int64)]),
+    ("freezeout_set_surface", C.c_int, [_VP, _DP, _DP, _DP]),
+    ("freezeout_pion_yield", C.c_int, [_VP, C.POINTER(CfOpts), _DP]),
 )
 
 

@@ -68,6 +68,9 @@ class Solver:
         return s
 
     def close(self):
+        # trackers bound to this context (fivo2d.freezeout) go first
+        for t in list(getattr(self, "_trackers", ())):
+            t.close()
         if getattr(self, "ctx", None) is not None and self.ctx.value and getattr(self, "_owned", True):
             self.lib.destroy(self.ctx)
         self.ctx = None

@@ -344,6 +344,51 @@ int fivo_batch_run(fivo_batch* b, const fivo_run_opts* o, const double* tau0, fi
 /* fivo_set_learned_seed for every event of the ensemble */
 int fivo_batch_set_learned_seed(fivo_batch* b, const fivo_mlp* m);
 
+/* ---- freeze-out surface and Cooper-Frye yield (freezeout2d.jl) ------------
+ * A tracker bound to one context records, after each primitive pass the caller
+ * makes, the last downward crossing of T_fo in every interior cell: the
+ * surface tau = tau_fo(x, y) and the flow (u^x, u^y) there (freeze_surface's
+ * record!, freezeout2d.jl:97-119). Cooper-Frye then integrates
+ *   dN/dy = f_yield / (2 pi hbar c)^3 sum dSigma_mu p^mu f(E_s / T_fo),
+ *   dSigma_mu p^mu = tau_fo (m_T cosh eta - d_x tau_fo p_x - d_y tau_fo p_y),
+ *   E_s = u^tau m_T cosh eta - u^x p_x - u^y p_y,
+ * over the cells 1..nx-2 x 1..ny-2 (0-based) with a frozen tau_fo and finite
+ * gradients (central differences, one-sided where one neighbour has none) on
+ * a p_T x phi x eta product grid, with Boltzmann and Bose statistics
+ * (pion_yield, freezeout2d.jl:144-192). Surface arrays are nx*ny doubles in
+ * the grid's interior order, index a*ny + b (x index a, y index b, 0-based). */
+typedef struct fivo_freezeout fivo_freezeout;
+typedef struct fivo_cf_opts {
+    int32_t n_eta, n_pt, n_phi;   /* grid sizes, each in 2..256 (40, 40, 24) */
+    int32_t _pad;
+    double eta_max;               /* eta in [-eta_max, eta_max] (6.0) */
+    double pt_lo, pt_hi;          /* GeV (0.1, 5.0) */
+    double mass;                  /* GeV (0.138, the pion) */
+    double yield_factor;          /* 2 for pi+ + pi-, 2.5 for feed-down (5.0) */
+} fivo_cf_opts;
+/* a tracker on ctx at T_fo (GeV): every cell not yet hot, no surface
+ * (freezeout2d.jl:87-91); the tracker must not outlive ctx */
+int fivo_freezeout_create(fivo_ctx* ctx, double T_fo, fivo_freezeout** out);
+void fivo_freezeout_destroy(fivo_freezeout* f);
+/* record! at tau from the context's current primitives (T = exp(yT), u^x,
+ * u^y), as left by the last fivo_rhs, fivo_update_primitives or fivo_run:
+ * T > T_fo marks the cell hot and clears its tau_fo (a cell that had frozen
+ * counts as re-heated); a hot cell at or below T_fo without a tau_fo freezes
+ * at the linear crossing between the previous sample and this one, with the
+ * flow of this sample (freezeout2d.jl:97-119) */
+int fivo_freezeout_record(fivo_freezeout* f, double tau);
+/* the surface: tau_fo (NaN where not frozen), u^x, u^y there (nullable each);
+ * counts[3] = cells ever hot, cells frozen, re-heating events (nullable)
+ * (freezeout2d.jl:127-132) */
+int fivo_freezeout_surface(fivo_freezeout* f, double* tau_fo, double* ux, double* uy, int64_t* counts);
+/* replace the surface (tau_fo NaN = not frozen); the hot flags and the
+ * re-heating count are left as they are */
+int fivo_freezeout_set_surface(fivo_freezeout* f, const double* tau_fo, const double* ux, const double* uy);
+/* pion_yield (freezeout2d.jl:144-192) on the current surface:
+ * out[3] = dN/dy Boltzmann, dN/dy Bose, the share of |dSigma.p| f_Boltzmann
+ * carried by dSigma.p < 0 */
+int fivo_freezeout_pion_yield(fivo_freezeout* f, const fivo_cf_opts* o, double* out);
+
 #ifdef __cplusplus
 }
 #endif
