@@ -397,6 +397,10 @@ double host_besselkx(int nu, double x) {
 }  // namespace
 
 // =============================================================================
+namespace fivo {
+void set_last_error(const std::string& msg) { g_err = msg; }
+}  // namespace fivo
+
 extern "C" {
 
 const char* fivo_last_error(const fivo_ctx* ctx) {

@@ -159,6 +159,8 @@ hipError_t launch_project_shear(Ctx& c, double* U, int correct);
 hipError_t launch_relax(Ctx& c, double* U, double tau, double dtau);
 // the relaxation's source terms at arbitrary states (which: 0 first moment, 1 second moment)
 hipError_t launch_probe(int which, uint32_t terms, int64_t n, const double* in, double* out, hipStream_t s);
+// the text fivo_last_error(NULL) returns, for errors without a context (fivo_api.hip)
+void set_last_error(const std::string& msg);
 hipError_t launch_layout(Ctx& c, const double* aos_dev, double* soa, int to_soa);
 // one slice of a transport background into column-major (Nx, Ny, *) tables at
 // element offset `off` (n may be null); maxima into sc->bg_{u,v}max_bits

@@ -20,6 +20,7 @@
 // =============================================================================
 #include <cmath>
 #include <limits>
+#include <mutex>
 #include <vector>
 #include "fivo_ctx.h"
 
@@ -224,10 +225,8 @@ __global__ __launch_bounds__(CF_BS) void k_cf_final(const double* __restrict__ p
     }
 }
 
-thread_local std::string g_fo_err;
-
 int fo_fail(Ctx* c, int code, const std::string& msg) {
-    if (c) c->err = msg; else g_fo_err = msg;
+    if (c) c->err = msg; else set_last_error(msg);
     return code;
 }
 #define FOCHK(c, x)                                                                         \
@@ -278,6 +277,52 @@ hipError_t launch_cooper_frye(const double* tfo, const double* uxf, const double
     hipLaunchKernelGGL(k_cf_final, dim3(1), dim3(CF_BS), 0, st, part, pref, out);
     return hipGetLastError();
 }
+
+// ---- the entry point that allocates nothing ------------------------------------
+// Its scratch -- the cell list, its length and the per-workgroup partials -- is
+// this file-scope device array, one copy per GPU, held until the process ends.
+// Host code never passes `scratch` itself to a kernel (on the host the name is a
+// stand-in, not a device address): it uses the address hipGetSymbolAddress
+// returns on each device, kept in g_scratch_dev[device].
+constexpr size_t NA_OFF_NLIST = 0;
+constexpr size_t NA_OFF_PART = 256;
+constexpr size_t NA_OFF_LIST = NA_OFF_PART + sizeof(double) * 4 * CF_NB;
+constexpr size_t NA_BYTES = NA_OFF_LIST + sizeof(int) * (size_t)FIVO_CF_NOALLOC_MAX_CELLS;
+static_assert(NA_OFF_LIST % 256 == 0, "scratch sections stay 256-byte aligned");
+
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+namespace {
+constexpr int NA_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[NA_MAX_DEVICES] = {};
+
+// the device address of `scratch` on the current device, looked up once per device
+int scratch_on_device(unsigned char** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) {
+        set_last_error(std::string("pion_yield_noalloc: hipGetDevice: ") + hipGetErrorString(e));
+        return FIVO_ERR_HIP;
+    }
+    if (dev < 0 || dev >= NA_MAX_DEVICES) {
+        set_last_error("pion_yield_noalloc: device index beyond 63");
+        return FIVO_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_last_error(std::string("pion_yield_noalloc: hipGetSymbolAddress: ") + hipGetErrorString(e));
+            return FIVO_ERR_HIP;
+        }
+        g_scratch_dev[dev] = p;
+    }
+    *out = static_cast<unsigned char*>(g_scratch_dev[dev]);
+    return FIVO_OK;
+}
+}  // namespace
 
 }  // namespace fivo
 
@@ -394,6 +439,35 @@ int fivo_freezeout_set_surface(fivo_freezeout* fp, const double* tau_fo, const d
     FOCHK(&c, hipMemcpyAsync(f.uxf, ux, n * sizeof(double), hipMemcpyHostToDevice, c.stream));
     FOCHK(&c, hipMemcpyAsync(f.uyf, uy, n * sizeof(double), hipMemcpyHostToDevice, c.stream));
     FOCHK(&c, hipStreamSynchronize(c.stream));
+    return FIVO_OK;
+}
+
+int fivo_pion_yield_noalloc(const double* tau_fo, const double* ux, const double* uy, int32_t nx, int32_t ny,
+                            double dx, double dy, double T_fo, const fivo_cf_opts* o, double* out, void* stream) {
+    if (!tau_fo || !ux || !uy || !out) return fo_fail(nullptr, FIVO_ERR_ARG, "pion_yield_noalloc: null array");
+    if (nx < 3 || ny < 3) return fo_fail(nullptr, FIVO_ERR_ARG, "pion_yield_noalloc: need nx, ny >= 3");
+    if ((int64_t)nx * ny > FIVO_CF_NOALLOC_MAX_CELLS)
+        return fo_fail(nullptr, FIVO_ERR_ARG,
+                       "pion_yield_noalloc: nx*ny = " + std::to_string((int64_t)nx * ny) + " exceeds the scratch limit " +
+                           std::to_string(FIVO_CF_NOALLOC_MAX_CELLS) + " cells");
+    if (!(dx > 0.0) || !(dy > 0.0)) return fo_fail(nullptr, FIVO_ERR_ARG, "pion_yield_noalloc: need dx, dy > 0");
+    CfGrid G;
+    double pref;
+    std::string err;
+    int rc = cf_grid(o, T_fo, dx, dy, G, pref, err);
+    if (rc != FIVO_OK) return fo_fail(nullptr, rc, err);
+    unsigned char* sp = nullptr;
+    if ((rc = scratch_on_device(&sp)) != FIVO_OK) return rc;
+    // every scratch word the kernels read is written first in the same call
+    // (k_cf_list writes the list and its length, k_cf_sum every partial), so
+    // nothing is cleared and the previous call's contents do not matter
+    const hipError_t e = launch_cooper_frye(tau_fo, ux, uy, nx, ny, dx, dy, G, pref,
+                                            reinterpret_cast<int*>(sp + NA_OFF_LIST),
+                                            reinterpret_cast<int*>(sp + NA_OFF_NLIST),
+                                            reinterpret_cast<double*>(sp + NA_OFF_PART), out,
+                                            static_cast<hipStream_t>(stream));
+    if (e != hipSuccess)
+        return fo_fail(nullptr, FIVO_ERR_HIP, std::string("pion_yield_noalloc: ") + hipGetErrorString(e));
     return FIVO_OK;
 }
 

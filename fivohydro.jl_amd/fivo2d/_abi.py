@@ -202,6 +202,9 @@ def load_hip():
                                 ("batch_run", C.c_int, [C.c_void_p, C.POINTER(RunOpts), _DP, C.POINTER(RunResult)]),
                                 ("init_hotspots", C.c_int, [C.c_void_p, C.c_double, C.c_int32, _DP, C.c_double,
                                                             C.c_double, C.c_double, C.c_int, C.POINTER(C.c_int64)]),
+                                ("pion_yield_noalloc", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32,
+                                                                 C.c_int32, C.c_double, C.c_double, C.c_double,
+                                                                 C.POINTER(CfOpts), C.c_void_p, C.c_void_p]),
                                 ("mlp_create", C.c_int, [C.c_int64, C.c_uint64, C.POINTER(C.c_void_p)]),
                                 ("mlp_destroy", None, [C.c_void_p]),
                                 ("mlp_last_error", C.c_char_p, [C.c_void_p]),

@@ -164,3 +164,34 @@ def freeze_surface(tag, N=200, box=18.0, dtau=0.02, taumax=16.0, eta_s=0.10, zet
 def pion_yield(s, n_eta=40, n_pt=40, n_phi=24, eta_max=6.0):
     """pion_yield (freezeout2d.jl:144-192) on the tracker's surface."""
     return s.pion_yield(n_eta=n_eta, n_pt=n_pt, n_phi=n_phi, eta_max=eta_max)
+
+
+NOALLOC_MAX_CELLS = 1 << 20
+
+
+def pion_yield_device(tau_fo, uxf, uyf, dx, dy, out, T_fo=T_FO, stream=None, lib=None, n_eta=40, n_pt=40, n_phi=24,
+                      eta_max=6.0, pt_lo=PT_LO, pt_hi=PT_HI, mass=M_PI, yield_factor=FYIELD):
+    """pion_yield on a surface already on the GPU, through the entry point that
+    allocates nothing (fivo_pion_yield_noalloc): tau_fo, uxf, uyf are contiguous
+    float64 CUDA tensors of shape (nx, ny), out a float64 CUDA tensor of 3
+    elements that receives (N_Boltzmann, N_Bose, negative fraction) on `stream`
+    (a torch.cuda.Stream; None: the current stream) without synchronising.
+    Calls that overlap on one GPU share its scratch and race."""
+    lib = lib if lib is not None else _abi.load_hip()
+    for a in (tau_fo, uxf, uyf, out):
+        if not (a.is_cuda and a.dtype.is_floating_point and a.element_size() == 8 and a.is_contiguous()):
+            raise ValueError("pion_yield_device: arrays must be contiguous float64 CUDA tensors")
+    nx, ny = tau_fo.shape
+    if tuple(uxf.shape) != (nx, ny) or tuple(uyf.shape) != (nx, ny) or out.numel() < 3:
+        raise ValueError("pion_yield_device: shapes do not match")
+    o = _abi.CfOpts()
+    o.n_eta, o.n_pt, o.n_phi = int(n_eta), int(n_pt), int(n_phi)
+    o.eta_max, o.pt_lo, o.pt_hi, o.mass, o.yield_factor = eta_max, pt_lo, pt_hi, mass, yield_factor
+    if stream is None:
+        import torch
+        stream = torch.cuda.current_stream()
+    rc = lib.pion_yield_noalloc(C.c_void_p(tau_fo.data_ptr()), C.c_void_p(uxf.data_ptr()),
+                                C.c_void_p(uyf.data_ptr()), int(nx), int(ny), float(dx), float(dy), float(T_fo),
+                                C.byref(o), C.c_void_p(out.data_ptr()), C.c_void_p(stream.cuda_stream))
+    if rc != 0:
+        raise FivoError(f"pion_yield_noalloc failed ({rc}): {lib.last_error(None)}")

@@ -389,6 +389,21 @@ int fivo_freezeout_set_surface(fivo_freezeout* f, const double* tau_fo, const do
  * carried by dSigma.p < 0 */
 int fivo_freezeout_pion_yield(fivo_freezeout* f, const fivo_cf_opts* o, double* out);
 
+/* fivo_freezeout_pion_yield on a surface already in device memory, allocating
+ * nothing. No counterpart in the reference: for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc). tau_fo, ux, uy: device arrays of nx*ny doubles in
+ * the tracker's order (NaN tau_fo = not frozen); dx, dy: cell sizes (fm);
+ * out: a device array of 3 doubles, written on `stream` (a hipStream_t, NULL =
+ * the default stream) without synchronising -- read it after the stream. The
+ * scratch (cell list and per-workgroup partials, 4.03 MiB) is one static device
+ * array per GPU, held until the process ends: nx*ny above
+ * FIVO_CF_NOALLOC_MAX_CELLS is refused with FIVO_ERR_ARG, and calls that
+ * overlap on one GPU race on it. Errors: fivo_last_error(NULL). */
+#define FIVO_CF_NOALLOC_MAX_CELLS (1 << 20)
+int fivo_pion_yield_noalloc(const double* tau_fo, const double* ux, const double* uy, int32_t nx, int32_t ny,
+                            double dx, double dy, double T_fo, const fivo_cf_opts* o, double* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
