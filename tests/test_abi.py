@@ -62,7 +62,8 @@ int main(void) {
 def test_refused_knobs():
     with pytest.raises(ValueError):
         F.build_model_2d(diff_dt_coeff=0.5)
-    with pytest.raises(ValueError):
-        F.build_model_2d(advect_nu=True, relax_advect_nu=False)
+    with pytest.raises(ValueError, match="Pick ONE"):
+        F.build_model_2d(advect_nu=True)                          # relax_advect_nu defaults to true
+    F.build_model_2d(advect_nu=True, relax_advect_nu=False)       # flux transport alone is built
     with pytest.raises(ValueError):
         F.build_model_2d(enable_diff=True, consistent_fm=True, tauN_coeff=2.0)
