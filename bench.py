@@ -6,10 +6,11 @@ A step is one accepted time step of run_sim_2d! (main2D.jl:436-513): every RK
 stage's primitive pass, fused face/update kernel and ghost fill, plus (with
 dissipation) the primitive refresh and the relaxation. Inputs are resident in
 HBM when the timed region starts. `value` is cell-updates/s = steps * Nx*Ny /
-wall time (bench/bench2d.jl:52). `value` is the batched+learned path on the
-workload's event when its final T stays within 1e-9 of the plain path's on the
-same steps (the network is trained on other events before the timed regions),
-else the plain path; every leg is reported. Prints ONE JSON line.
+wall time (bench/bench2d.jl:52). `value` is the fastest of the plain,
+plain+learned and batched+learned paths on the workload's event whose final T
+stays within 1e-9 of the plain path's on the same steps (the network is trained
+on other events before the timed regions); every leg is reported. Prints ONE
+JSON line.
 
 roofline: the plain path's dominant kernel's algorithmic HBM bytes per launch (the per-cell
 model of the workload's bytes/cell-update, restricted to that kernel) over its
@@ -166,6 +167,7 @@ def train_warm_start(lib, target_pairs=10_000_000, passes=24):
         s.close()
     t_collect = time.perf_counter() - t0
     net.fit_normalization()
+    mse0 = net.eval()
     t0 = time.perf_counter()
     losses = []
     for epochs, lr in ((8, 3e-3), (6, 1e-3), (3, 3e-4)):
@@ -173,10 +175,17 @@ def train_warm_start(lib, target_pairs=10_000_000, passes=24):
     t_train = time.perf_counter() - t0
     log(f"warm start: {net.npairs} pairs in {t_collect:.1f} s ({per_cfg}), trained in {t_train:.1f} s")
     mse = net.eval()
+    outs = ("yT", "phi", "asinh_us", "u_perp", "log_e", "log_P")
+    # u_perp (the flow across the momentum, zero up to the cells' noise) is not
+    # learnable and dominates the summed loss, so progress is reported per output
+    # and summed without it
+    learn = [k for k, o in enumerate(outs) if o != "u_perp"]
     info = {"pairs": net.npairs, "pairs_by_configuration": per_cfg, "collect_s": round(t_collect, 3),
             "train_s": round(t_train, 3), "epochs": len(losses),
             "loss_first_last": [float(losses[0]), float(losses[-1])],
-            "mse_normalised": {k: float(v) for k, v in zip(("yT", "phi", "asinh_us", "u_perp", "log_e", "log_P"), mse)},
+            "mse_normalised_before_training": {k: float(v) for k, v in zip(outs, mse0)},
+            "mse_normalised": {k: float(v) for k, v in zip(outs, mse)},
+            "mse_without_u_perp_before_after": [float(sum(mse0[k] for k in learn)), float(sum(mse[k] for k in learn))],
             "training_events": "configuration 2 (6 x 256^2, SSPRK2), 3 (3 x 512^2) and 4 (6 events 128-384^2), "
                                "seeds 7100-7305, a pair set every two steps"}
     return net, info
@@ -249,38 +258,69 @@ def background_bench(lib, s, g):
     return out
 
 
-def conditioning_check(lib, e, evs, m, grids, leg, steps, warmup, Tref, OKref, flagged):
-    """The events whose batched+learned run leaves 1e-9 of the plain run (or flips
-    a recovery flag), rerun on the oracle from the same initial state and steps:
-    an event whose plain HIP run and oracle run (round-off apart) also part by more
-    than 1e-9 or flip a flag is perturbation-sensitive in the reference algorithm
-    itself, the learned seed merely one more perturbation."""
+def conditioning_check(lib, e, evs, m, grids, leg, steps, warmup, Tref, OKref, flagged, net, nperturb=2):
+    """Each event whose batched+learned run leaves 1e-9 of the plain run (or flips a
+    recovery flag), classified two ways on the plain run's final state and steps:
+
+    root check -- the final state recovered from a full cold start (every warm
+      start cleared) with the reference's cold seed and with the learned seed:
+      the largest per-cell dT/T and flag flips between the two. Round-off-level
+      values mean the learned seed reaches the cold seed's root on every cell.
+    perturbation -- the plain run repeated from the initial state with every
+      conserved entry scaled by 1 +- 1e-13 (`nperturb` sign patterns): an event
+      whose own plain run parts by more than 1e-9 or flips a flag under such a
+      perturbation is perturbation-sensitive in the reference algorithm, and the
+      learned seed's round-off-level difference is one more such perturbation."""
     import fivo2d as F
-    orc = F.Lib(os.path.join(REPO, "oracle", "liboracle2d.so"), "orc_")
-    orc.dll.orc_set_threads.restype = C.c_int
-    orc.dll.orc_set_threads.argtypes = [C.c_int]
-    orc.dll.orc_set_threads(os.cpu_count() or 1)
-    sensitive = []
+    keys = ("x0_yT", "x0_phi", "x0_ux", "x0_uy")
+    rows = []
     t0 = time.perf_counter()
     for k in flagged:
         n, nsp, seed = evs[k]
-        sh = F.Solver(grids[k], m, lib=lib)
-        sh.init_hotspots(e["tau0"], seed=seed, nspots=nsp, sigma=e["sigma"], Tpeak=0.45)
-        so = F.Solver(grids[k], m, lib=orc)
-        so.set_state(sh.get_state())
-        so.set_work(sh.get_work())
-        sh.close()
-        r = so.run_sim_2d(tau0=e["tau0"], max_steps=warmup, integrator="ssprk3", **leg)
-        so.run_sim_2d(tau0=r["tau"], max_steps=steps, integrator="ssprk3", reset_history=False, **leg)
-        w = so.get_work(("yT", "ok"))
-        so.close()
-        both = (w["ok"] == 1) & (OKref[k] == 1)
-        d = float(np.max(np.where(both, np.abs(np.exp(w["yT"]) - Tref[k]) / Tref[k], 0.0)))
-        fl = int(np.sum(w["ok"] != OKref[k]))
-        sensitive.append({"event": k, "plain_vs_oracle_max_dT_over_T": d, "plain_vs_oracle_flips": fl,
-                          "sensitive": bool(d > 1e-9 or fl > 0)})
-    return {"flagged_events": sensitive,
-            "flagged_events_not_sensitive": sum(1 for x in sensitive if not x["sensitive"]),
+        g = grids[k]
+        # the plain run again, one event (bit-identical to its ensemble run)
+        P = F.Solver(g, m, lib=lib)
+        P.init_hotspots(e["tau0"], seed=seed, nspots=nsp, sigma=e["sigma"], Tpeak=0.45)
+        U0 = P.get_state()
+        r = P.run_sim_2d(tau0=e["tau0"], max_steps=warmup + steps, integrator="ssprk3", **leg)
+        # root check on its final state
+        U, W = P.get_state(), P.get_work()
+        rec = []
+        for learned in (False, True):
+            t = F.Solver(g, m, lib=lib)
+            if learned:
+                t.set_learned_seed(net)
+            t.set_state(U)
+            t.set_work(dict(W, **{kk: np.full_like(W[kk], np.nan) for kk in keys}))
+            t.update_primitives_2d(r["tau"])
+            rec.append(t.get_work(("yT", "ok", "ux", "uy")))
+            t.close()
+        ok2 = (rec[0]["ok"] == 1) & (rec[1]["ok"] == 1)
+        Ta = np.exp(rec[0]["yT"])
+        root_dT = float(np.max(np.where(ok2, np.abs(np.exp(rec[1]["yT"]) - Ta) / Ta, 0.0)))
+        root_flips = int(np.sum(rec[0]["ok"] != rec[1]["ok"]))
+        P.close()
+        # perturbed plain runs
+        pert = []
+        for ps in range(nperturb):
+            Q = F.Solver(g, m, lib=lib)
+            Q.init_hotspots(e["tau0"], seed=seed, nspots=nsp, sigma=e["sigma"], Tpeak=0.45)
+            sign = np.random.default_rng(100 + ps).choice([-1.0, 1.0], size=U0.shape)
+            Q.set_state(U0 * (1.0 + 1e-13 * sign))
+            Q.run_sim_2d(tau0=e["tau0"], max_steps=warmup + steps, integrator="ssprk3", **leg)
+            w = Q.get_work(("yT", "ok"))
+            Q.close()
+            both = (w["ok"] == 1) & (OKref[k] == 1)
+            pert.append((float(np.max(np.where(both, np.abs(np.exp(w["yT"]) - Tref[k]) / Tref[k], 0.0))),
+                         int(np.sum(w["ok"] != OKref[k]))))
+        rows.append({"event": k, "root_check_max_dT_over_T": root_dT, "root_check_flips": root_flips,
+                     "perturbed_1e-13_max_dT_over_T": [d for d, _ in pert],
+                     "perturbed_1e-13_flips": [f for _, f in pert],
+                     "sensitive": bool(any(d > 1e-9 or f > 0 for d, f in pert))})
+    return {"flagged_events": rows,
+            "flagged_events_not_sensitive": sum(1 for x in rows if not x["sensitive"]),
+            "flagged_events_other_root": sum(1 for x in rows
+                                             if x["root_check_max_dT_over_T"] > 1e-10 or x["root_check_flips"] > 0),
             "conditioning_check_s": time.perf_counter() - t0}
 
 
@@ -353,7 +393,7 @@ def ensemble_bench(lib, name, steps, warmup, net):
             info["learned_starts"] = nl
             info["fallback_rate"] = sum(r["nlearned_fallback"] for r in res) / max(nl, 1)
             info.update(conditioning_check(lib, e, evs, m, grids, leg, steps, warmup, Tref, OKref,
-                                           [k for k, b in enumerate(bad) if b]))
+                                           [k for k, b in enumerate(bad) if b], net))
         out[path] = info
         log(f"{name} {path}: {info['cell_updates_per_s']:.3e} cell-updates/s, max dT/T {dT:.1e}")
         x.close()
@@ -611,12 +651,13 @@ def main():
             bl = batched_learned_single(cfg, lib, net, a.steps, max(a.warmup, 1), Tp, wall)
             out["batched_learned_512"] = bl
             log(f"{cfg['workload']} batched+learned: {bl['cell_updates_per_s']:.3e}")
-            # the headline is the batched+learned path when its final T stays within
-            # 1e-9 of the plain path's on the same steps, else the plain path
-            if bl["max_dT_over_T_vs_plain"] <= 1e-9:
-                out["value"] = bl["cell_updates_per_s"]
-                out["ms_per_step"] = bl["ms_per_step"]
-                out["config"]["path"] = "batched+learned"
+            # the headline is the fastest path whose final T stays within 1e-9 of
+            # the plain path's on the same steps (the plain path always qualifies)
+            for pname, leg in (("plain+learned", out["plain_learned_512"]), ("batched+learned", bl)):
+                if leg["max_dT_over_T_vs_plain"] <= 1e-9 and leg["cell_updates_per_s"] > out["value"]:
+                    out["value"] = leg["cell_updates_per_s"]
+                    out["ms_per_step"] = leg["ms_per_step"]
+                    out["config"]["path"] = pname
         for e in ("c4", "c5"):
             if e in legs:
                 out[f"ensemble_{e}"] = ensemble_bench(lib, e, a.ensemble_steps, 3, net)
@@ -634,6 +675,7 @@ def main():
             tg[f"ensemble_{e}_events_with_dT_over_1e-9_or_flips"] = bl["events_with_cells_dT_over_1e-9_or_flips"]
             tg[f"ensemble_{e}_max_dT_over_T_other_events"] = bl["max_dT_over_T_other_events"]
             tg[f"ensemble_{e}_flagged_events_not_perturbation_sensitive"] = bl["flagged_events_not_sensitive"]
+            tg[f"ensemble_{e}_flagged_events_on_another_root"] = bl["flagged_events_other_root"]
     out["targets"] = tg
     if "background" in legs:
         out["background_export"] = background_bench(lib, s, g)
