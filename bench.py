@@ -12,18 +12,23 @@ stays within 1e-9 of the plain path's on the same steps (the network is trained
 on other events before the timed regions); every leg is reported. Prints ONE
 JSON line.
 
-roofline: the plain path's dominant kernel's algorithmic HBM bytes per launch (the per-cell
-model of the workload's bytes/cell-update, restricted to that kernel) over its
-average duration from HIP events on the library's stream, recorded over the K
-steps that follow the timed region (one event pair around each launch serialises
-the face tiles that otherwise overlap the cooperative primitive kernel, so the
-timed region runs without them).
+roofline: the plain path's dominant kernel's algorithmic HBM bytes per launch (the
+per-array model of kernel_alg_bytes) over its average duration from HIP events on
+the library's stream, recorded over the K steps that follow the timed region (one
+event pair around each launch serialises the face tiles that otherwise overlap
+the cooperative primitive kernel, so the timed region runs without them).
+`per_kernel` puts every timed kernel against both rooflines, HBM (8 TB/s) and
+FP64 (78.6 TFLOP/s, FP64 FLOP per launch a profile constant), and names the
+binding one; `step_hbm_frac` sums the model over a step's kernels. `traffic` is a
+profile constant, labelled as such (`traffic_source`).
 cpu_baseline: the project's FP64 C++ restatement of the reference solver (the
 parity oracle), OpenMP over x-columns on the host cores, on a bounded sample of
 the same workload.
 background_export: the readout row (tools/export_background2d.jl) at its
 defaults, end to end on the GPU and on the oracle, and its slice kernel's HBM
 rate on the 512^2 state.
+freezeout: freezeout2d.jl's axisymmetric control at N = 300 (surface and
+Cooper-Frye on the GPU; the same surface's Cooper-Frye on the oracle).
 """
 import argparse
 import ctypes as C
@@ -44,17 +49,24 @@ def log(msg):
     """Progress of the legs on stderr; the result is the one JSON line on stdout."""
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
-# HBM bytes per launch of the dominant kernel measured with rocprofv3 --pmc
-# FETCH_SIZE / WRITE_SIZE, scaled by the same counters on a device copy of known
-# size (profiles/full_sectors_512_kernel_stats.md); keyed by (config, kernel).
-# The figures are those of the serialised launches the HIP events time
-# (cooperative kernel [alone], face kernel [one pass]).
+# Profile constants, not measured by this run (a counter pass needs its own
+# rocprofv3 run): HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE on this workload, scaled by the same counters on a device copy of
+# known size, and FP64 FLOP per launch from the SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64
+# counters (profiles/full_sectors_512_kernel_stats.md, with the commands that made
+# them); keyed by (config, kernel). The figures are those of the serialised
+# launches the HIP events time (cooperative kernel [alone], face kernel [one pass]).
 TRAFFIC = {
     ("c3", "k_prim_coop"): (2.0 * 1076.2 + 485.2) * 1024,
     ("c3", "k_prim"): (2.0 * 15272.0 + 33812.5) * 1024,
     ("c3", "k_faces"): (2.0 * 41906.4 + 34280.4) * 1024,
     ("c3", "k_relax"): (2.0 * 45060.3 + 67090.8) * 1024,
 }
+
+TRAFFIC_SOURCE = ("profile constant: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this workload, "
+                  "scaled by a 512 MiB device copy (profiles/full_sectors_512_kernel_stats.md)")
+FP64_GFLOP = {}
+FP64_PEAK_TFLOPS = 78.6
 
 # Workloads: (description, builder). Algorithmic HBM bytes per cell-update and per
 # dominant-kernel launch follow the per-array model: every array of a pass is read
@@ -204,6 +216,69 @@ def _bg_event_seed(g, seed):
 
 def _bg_event(g):
     return _bg_event_seed(g, 20)
+
+
+FO_WORKLOAD = ("freezeout2d.jl's axisymmetric control: the radial profile initial_profiles_physical.csv "
+               "on 300x300 over [-18,18] fm, shear + bulk + charge, tau 0.4 -> 16 fm/c in 0.02 fm/c chunks "
+               "(a primitive pass and a record! after each), then pion_yield on the reference's "
+               "40 p_T x 24 phi x 40 eta grid")
+
+
+def freezeout_bench(lib, cpu=True):
+    """The freeze-out row end to end on the GPU (freeze_surface: evolution plus a
+    device record! per chunk; pion_yield: the Cooper-Frye sweep), and the same
+    surface integrated by the oracle on the host cores."""
+    import torch
+    import fivo2d as F
+    from fivo2d import freezeout as FO
+    csv = os.path.join(REPO, "tests", "golden", "initial_profiles_physical.csv")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fo = FO.freeze_surface("axisym", N=300, dtau=0.02, lib=lib, profile_csv=csv, verbose=False)
+    torch.cuda.synchronize()
+    t_surf = time.perf_counter() - t0
+    fo.pion_yield()                                   # warm-up
+    reps = 5
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        y = fo.pion_yield()
+    torch.cuda.synchronize()
+    t_cf = (time.perf_counter() - t0) / reps
+    sf = fo.surface()
+    ncell = int(np.sum(~np.isnan(sf["tau_fo"][1:-1, 1:-1])))
+    points = ncell * 40 * 24 * 40
+    out = {"workload": FO_WORKLOAD, "surface_s": t_surf, "chunks": int(round((16.0 - 0.4) / 0.02)),
+           "cells_hot": sf["nhot"], "cells_frozen": sf["nfroze"], "reheated": sf["refroze"],
+           "dNdy_boltzmann": y[0], "dNdy_bose": y[1], "negative_fraction": y[2],
+           "cooper_frye_ms": t_cf * 1e3, "cooper_frye_points": points,
+           "cooper_frye_points_per_s": points / t_cf}
+    if cpu:
+        orc = F_oracle()
+        s2 = F.Solver(fo.solver.g, fo.solver.model, lib=orc)
+        t2 = FO.FreezeOut(s2)
+        t2.set_surface(sf["tau_fo"], sf["uxf"], sf["uyf"])
+        t0 = time.perf_counter()
+        yc = t2.pion_yield()
+        t_cpu = time.perf_counter() - t0
+        out["cpu_oracle"] = {"cooper_frye_s": t_cpu, "cores": orc.dll.orc_set_threads(0),
+                             "max_rel_diff_vs_gpu": max(abs(yc[0] / y[0] - 1), abs(yc[1] / y[1] - 1)),
+                             "speedup_gpu_vs_cpu": t_cpu / t_cf}
+        t2.close()
+        s2.close()
+    fo.solver.close()
+    log(f"freeze-out: surface {t_surf:.2f} s, Cooper-Frye {t_cf * 1e3:.2f} ms ({points / t_cf:.3e} points/s), "
+        f"dN/dy Bose {y[1]:.2f}")
+    return out
+
+
+def F_oracle():
+    import fivo2d as F
+    orc = F.Lib(os.path.join(REPO, "oracle", "liboracle2d.so"), "orc_")
+    orc.dll.orc_set_threads.restype = C.c_int
+    orc.dll.orc_set_threads.argtypes = [C.c_int]
+    orc.dll.orc_set_threads(os.cpu_count() or 1)
+    return orc
 
 
 def background_bench(lib, s, g):
@@ -419,23 +494,67 @@ def faces_bytes_per_launch(cfg, ncell):
 
 
 def kernel_alg_bytes(kernel, cfg, g, r, launches):
-    """Algorithmic HBM bytes of one launch of `kernel`, from the per-array model:
-    faces: the stage's RK state (2 nv doubles in stage 1, 3 nv in the weighted
-    stages) per interior cell; warm primitive kernel: the Newton warm starts, 8
-    doubles per cell; cooperative kernel: per queued cell the state (nv) and warm
-    start (4) read, primitives (11) and warm start (4) written; relaxation: U read,
-    dissipative rows written, histories read and written."""
+    """Algorithmic HBM bytes of one launch of `kernel`, from the per-array model
+    (every array the kernel needs is read or written once per cell it covers):
+      k_prim       every cell (ghosts included): U read (nv), warm starts read and
+                   written (4 + 4), primitives written (yT, phi, mu, alpha, u^x, u^y,
+                   P, n, e, v^x, v^y: 11) and the ok byte
+      k_prim_coop  per queued cell: the same (nv + 19 doubles + 1 byte)
+      k_faces      per interior cell: the stage's RK state (2 nv doubles in stage 1,
+                   3 nv in the weighted stages, averaged), the primitives the
+                   reconstruction and the admissibility read (yT, phi, u^x, u^y, P,
+                   v^x, v^y: 7) and the ok / MOOD bytes (2)
+      k_relax      per interior cell: U read and the next buffer written (2 nv), the
+                   primitives read (yT, alpha, mu, n, e, P, u^x, u^y: 8 and ok), the
+                   six histories read and written (12)
+      k_bc         per ghost cell: nv doubles read and written"""
     ncell = g.Nx * g.Ny
     nv = cfg["nv"]
     if kernel == "k_faces":
-        return faces_bytes_per_launch(cfg, ncell)
+        return faces_bytes_per_launch(cfg, ncell) + (8 * 7 + 2) * ncell
     if kernel == "k_prim":
-        return 8 * 8 * g.Ntot
+        return (8 * (nv + 19) + 1) * g.Ntot
     if kernel == "k_prim_coop":
-        return 8 * (nv + 19) * r["nqueued"] / max(launches, 1)
+        return (8 * (nv + 19) + 1) * r["nqueued"] / max(launches, 1)
     if kernel == "k_relax":
-        return 8 * (nv + (nv - 4) + 12) * ncell
+        return (8 * (2 * nv + 8 + 12) + 1) * ncell
+    if kernel == "k_bc":
+        return 8 * 2 * nv * (g.Ntot - ncell)
     return 8 * nv * 2 * ncell
+
+
+def per_kernel_roofline(tms, cfg, g, r, steps, config_key):
+    """Each timed kernel against both of its rooflines: algorithmic HBM bytes per
+    launch over its HIP-event average (fraction of 8 TB/s), and FP64 FLOP per
+    launch (profile constant) over the same average (fraction of 78.6 TFLOP/s);
+    the larger fraction names the binding roofline, and a kernel far below both
+    is latency-bound."""
+    out = {}
+    step_bytes = 0.0
+    step_flop = 0.0
+    for k, (tot_ms, n) in tms.items():
+        avg = tot_ms / n
+        alg = kernel_alg_bytes(k, cfg, g, r, n)
+        gbs = alg / (avg * 1e-3) / 1e9
+        row = {"avg_ms": round(avg, 5), "launches_per_step": n / steps, "alg_bytes_per_launch": alg,
+               "achieved_GBs": gbs, "hbm_frac": gbs / HBM_PEAK_GBS}
+        tr = TRAFFIC.get((config_key, k))
+        if tr is not None:
+            row["traffic_bytes_per_launch"] = tr
+            row["traffic_over_alg"] = tr / alg if alg else None
+        gf = FP64_GFLOP.get((config_key, k))
+        if gf is not None:
+            tf = gf / avg            # GFLOP / ms = TFLOP/s
+            row["fp64_gflop_per_launch"] = gf
+            row["fp64_TFLOPs"] = tf
+            row["fp64_frac"] = tf / FP64_PEAK_TFLOPS
+            step_flop += gf * 1e9 * n / steps
+        fr = max(row["hbm_frac"], row.get("fp64_frac", 0.0))
+        row["binding"] = ("latency" if fr < 0.2 else
+                          ("fp64" if row.get("fp64_frac", 0.0) > row["hbm_frac"] else "hbm"))
+        step_bytes += alg * n / steps
+        out[k] = row
+    return out, step_bytes, step_flop
 
 
 def make_inputs(cfg):
@@ -553,8 +672,8 @@ def main():
     ap.add_argument("--config", default="c3", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true", help="plain path only (no network, no ensembles)")
-    ap.add_argument("--legs", default="learned512,c4,c5,background",
-                    help="comma list of the legs beside the plain one: learned512, c4, c5, background")
+    ap.add_argument("--legs", default="learned512,c4,c5,background,freezeout",
+                    help="comma list of the legs beside the plain one: learned512, c4, c5, background, freezeout")
     ap.add_argument("--ensemble-steps", type=int, default=10)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a device copy of 512 MiB (the reference kernel of the traffic counters)")
@@ -595,6 +714,8 @@ def main():
     avg_ms = tms[dom][0] / tms[dom][1]
     alg = kernel_alg_bytes(dom, cfg, g, rk, tms[dom][1])
     achieved = alg / (avg_ms * 1e-3) / 1e9
+    pk, step_bytes, step_flop = per_kernel_roofline(tms, cfg, g, rk, a.steps, a.config)
+    ms_k = wall_k / a.steps
     out = {
         "metric": "cell-updates/s, plain vs batched+learned, 512² viscous+charm, ΔT≤1e-9",
         "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
@@ -609,7 +730,15 @@ def main():
                    "kernel_launches": {k: v[1] for k, v in tms.items()}},
         "roofline": {"bound": "hbm", "kernel": dom, "path": "plain", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC.get((a.config, dom)),
+                     "traffic_source": TRAFFIC_SOURCE if (a.config, dom) in TRAFFIC else None,
                      "alg_bytes_per_launch": alg, "avg_launch_ms": avg_ms,
+                     "binding": pk[dom]["binding"],
+                     "per_kernel": pk,
+                     # the whole step: the per-kernel model summed over the kernels a step
+                     # launches, over the step time measured with the same events on
+                     "step_alg_bytes": step_bytes, "step_hbm_frac": step_bytes / (ms_k * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "step_fp64_frac": (step_flop / (ms_k * 1e-3) / 1e12 / FP64_PEAK_TFLOPS) if step_flop else None,
+                     # the survey's per-cell-update model (a lower bound: no primitives)
                      "step_achieved_GBs": algorithmic_bytes(cfg) * value / 1e9},
     }
     if a.copy_calibration:
@@ -665,7 +794,8 @@ def main():
     # the stated targets, read off the legs above: batched+learned against the plain
     # path on the ensembles (with their final-time differences), and the plain 512^2
     # step's algorithmic HBM rate as a fraction of the peak
-    tg = {"plain_step_hbm_fraction": out["roofline"]["step_achieved_GBs"] / HBM_PEAK_GBS}
+    tg = {"plain_step_hbm_fraction": out["roofline"]["step_hbm_frac"],
+          "plain_step_hbm_fraction_survey_model": out["roofline"]["step_achieved_GBs"] / HBM_PEAK_GBS}
     for e in ("c4", "c5"):
         x = out.get(f"ensemble_{e}")
         if x:
@@ -679,6 +809,8 @@ def main():
     out["targets"] = tg
     if "background" in legs:
         out["background_export"] = background_bench(lib, s, g)
+    if "freezeout" in legs:
+        out["freezeout"] = freezeout_bench(lib, cpu=not a.no_cpu)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cfg, background="background" in legs)
         log(f"cpu baseline: {out['cpu_baseline']['value']:.3e} cell-updates/s")

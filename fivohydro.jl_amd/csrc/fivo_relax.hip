@@ -212,7 +212,10 @@ __device__ __forceinline__ void m2_coeffs(M2In& a, double Tm, double z, double K
 }
 
 // consistent_m2_source_2d (src2d/hq_consistent_m2_2d.jl:247-427): the four rates.
-// Out of line: the update evaluates it five times.
+// Out of line: the update evaluates it five times. INST separates the copy the
+// relaxation calls (0) from the probe's (1): the same source, but a second caller
+// of one out-of-line copy costs the relaxation kernel 224 bytes of scratch per lane.
+template <int INST>
 __device__ __noinline__ void m2_source(const Params& p, const M2In& a, double pxx, double pxy, double pyy, double PiQ,
                                        double out[4]) {
     const double Tm = jmax(a.T, T_MIN);
@@ -526,13 +529,13 @@ __device__ __forceinline__ void relax_cell(const double* __restrict__ Q, double*
                         a.n = n; a.taun = taun; a.tauM = tMq; a.etaM = jmax(T, T_MIN) * taun / 2;
                         m2_coeffs(a, Tm3, z, K2, K3, K4, mq);
                         double s0[4], sp[4], cc[4];
-                        m2_source(p, a, pq[0], pq[1], pq[2], pq[4], s0);
+                        m2_source<0>(p, a, pq[0], pq[1], pq[2], pq[4], s0);
                         const double A2 = safe_div(tMq * ut, D);
                         const double dp = 1e-7 * jmax(jmax(jmax(jmax(1.0, fabs(pq[0])), fabs(pq[1])), fabs(pq[2])), fabs(pq[4]));
-                        m2_source(p, a, pq[0] + dp, pq[1], pq[2], pq[4], sp); cc[0] = -(sp[0] - s0[0]) / dp * tMq * ut;
-                        m2_source(p, a, pq[0], pq[1] + dp, pq[2], pq[4], sp); cc[1] = -(sp[1] - s0[1]) / dp * tMq * ut;
-                        m2_source(p, a, pq[0], pq[1], pq[2] + dp, pq[4], sp); cc[2] = -(sp[2] - s0[2]) / dp * tMq * ut;
-                        m2_source(p, a, pq[0], pq[1], pq[2], pq[4] + dp, sp); cc[3] = -(sp[3] - s0[3]) / dp * tMq * ut;
+                        m2_source<0>(p, a, pq[0] + dp, pq[1], pq[2], pq[4], sp); cc[0] = -(sp[0] - s0[0]) / dp * tMq * ut;
+                        m2_source<0>(p, a, pq[0], pq[1] + dp, pq[2], pq[4], sp); cc[1] = -(sp[1] - s0[1]) / dp * tMq * ut;
+                        m2_source<0>(p, a, pq[0], pq[1], pq[2] + dp, pq[4], sp); cc[2] = -(sp[2] - s0[2]) / dp * tMq * ut;
+                        m2_source<0>(p, a, pq[0], pq[1], pq[2], pq[4] + dp, sp); cc[3] = -(sp[3] - s0[3]) / dp * tMq * ut;
                         const bool ram = p.relax_advect_m2 != 0;
                         const int row[4] = {p.iPQ0, p.iPQ0 + 1, p.iPQ0 + 2, p.iPQ0 + 4};
                         const int fld[4] = {0, 1, 2, 4};
@@ -651,7 +654,7 @@ __global__ void k_probe_m2(Params p, int64_t n, const double* __restrict__ in, d
     if (!(a.tauM > 0.0)) return;
     const double Tm = jmax(a.T, T_MIN), z = m / Tm;
     m2_coeffs(a, Tm, z, safe_besselkx(2, z), safe_besselkx(3, z), safe_besselkx(4, z), m);
-    m2_source(p, a, v[10], v[11], v[12], v[14], o);
+    m2_source<1>(p, a, v[10], v[11], v[12], v[14], o);
 }
 
 hipError_t launch_probe(int which, uint32_t terms, int64_t n, const double* in, double* out, hipStream_t s) {
