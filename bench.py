@@ -57,15 +57,17 @@ def log(msg):
 # them); keyed by (config, kernel). The figures are those of the serialised
 # launches the HIP events time (cooperative kernel [alone], face kernel [one pass]).
 TRAFFIC = {
-    ("c3", "k_prim_coop"): (2.0 * 1076.2 + 485.2) * 1024,
-    ("c3", "k_prim"): (2.0 * 15272.0 + 33812.5) * 1024,
-    ("c3", "k_faces"): (2.0 * 41906.4 + 34280.4) * 1024,
-    ("c3", "k_relax"): (2.0 * 45060.3 + 67090.8) * 1024,
+    ("c3", "k_prim_coop"): (2.0 * 1131.6 + 1509.9) * 1024,
+    ("c3", "k_prim"): (2.0 * 15274.2 + 33818.9) * 1024,
+    ("c3", "k_faces"): (2.0 * 42034.6 + 34280.4) * 1024,
+    ("c3", "k_relax"): (2.0 * 45013.4 + 67113.7) * 1024,
+    ("c3", "k_bc"): (2.0 * 1254.5 + 1159.0) * 1024,
 }
 
 TRAFFIC_SOURCE = ("profile constant: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this workload, "
                   "scaled by a 512 MiB device copy (profiles/full_sectors_512_kernel_stats.md)")
-FP64_GFLOP = {}
+FP64_GFLOP = {("c3", "k_prim_coop"): 0.865, ("c3", "k_prim"): 0.909, ("c3", "k_faces"): 1.347,
+              ("c3", "k_relax"): 0.554, ("c3", "k_bc"): 0.0}
 FP64_PEAK_TFLOPS = 78.6
 
 # Workloads: (description, builder). Algorithmic HBM bytes per cell-update and per
@@ -715,7 +717,7 @@ def main():
     alg = kernel_alg_bytes(dom, cfg, g, rk, tms[dom][1])
     achieved = alg / (avg_ms * 1e-3) / 1e9
     pk, step_bytes, step_flop = per_kernel_roofline(tms, cfg, g, rk, a.steps, a.config)
-    ms_k = wall_k / a.steps
+    sec_k = wall_k / a.steps
     out = {
         "metric": "cell-updates/s, plain vs batched+learned, 512² viscous+charm, ΔT≤1e-9",
         "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
@@ -736,8 +738,8 @@ def main():
                      "per_kernel": pk,
                      # the whole step: the per-kernel model summed over the kernels a step
                      # launches, over the step time measured with the same events on
-                     "step_alg_bytes": step_bytes, "step_hbm_frac": step_bytes / (ms_k * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                     "step_fp64_frac": (step_flop / (ms_k * 1e-3) / 1e12 / FP64_PEAK_TFLOPS) if step_flop else None,
+                     "step_alg_bytes": step_bytes, "step_hbm_frac": step_bytes / sec_k / 1e9 / HBM_PEAK_GBS,
+                     "step_fp64_frac": (step_flop / sec_k / 1e12 / FP64_PEAK_TFLOPS) if step_flop else None,
                      # the survey's per-cell-update model (a lower bound: no primitives)
                      "step_achieved_GBs": algorithmic_bytes(cfg) * value / 1e9},
     }
