@@ -58,6 +58,23 @@ __device__ __forceinline__ double safe_div(double a, double b) { return a / posd
 __device__ __forceinline__ double safe_sqrt(double x) { return x <= 0 ? 0.0 : sqrt(x); }
 __device__ __forceinline__ double pow4(double x) { double x2 = x * x; return x2 * x2; }
 __device__ __forceinline__ double pow3(double x) { return x * x * x; }
+// max of N values none of which is NaN, as a tree of v_max: exact and independent of
+// the order, so it equals any sequential `max` loop over the same values
+template <int N>
+__device__ __forceinline__ double max_tree(const double* v) {
+    if constexpr (N == 1) return v[0];
+    else return fmax(max_tree<N / 2>(v), max_tree<N - N / 2>(v + N / 2));
+}
+// Julia's `m = 0.0; for r: m = max(m, |x_r|)` over non-negative |x_r|: NaN when
+// any is NaN (a sum carries it beside the tree), the tree's max otherwise
+template <int N>
+__device__ __forceinline__ double max_abs_nan(const double* v) {
+    double a[N], s = 0.0;
+#pragma unroll
+    for (int r = 0; r < N; ++r) { a[r] = fabs(v[r]); s += a[r]; }
+    const double m = max_tree<N>(a);
+    return isnan_(s) ? s : m;
+}
 __device__ __forceinline__ double dinf() { return __longlong_as_double(0x7ff0000000000000LL); }
 
 // mc_limiter (src/utils.jl:14-22)
@@ -432,20 +449,41 @@ __device__ __forceinline__ bool solve_gauss(double J[4][4], const double F[4], d
         for (int c = 0; c < N; ++c) A[r][c] = J[r][c];
         A[r][N] = -F[r];
     }
-    double smax = 0.0;
+    // smax = max |A_rc| (Julia's loop of `max`, which fails the check below on any
+    // NaN or Inf): a finiteness test and a tree of v_max, exact and order-free for
+    // finite values, in place of a 16-deep chain of NaN-propagating maxima
+    double aa[N * N];
+    bool fin_in = true;
 #pragma unroll
     for (int r = 0; r < N; ++r)
 #pragma unroll
-        for (int c = 0; c < N; ++c) smax = jmax(smax, fabs(A[r][c]));
-    if (!isf(smax) || smax == 0.0) return false;
+        for (int c = 0; c < N; ++c) { aa[r * N + c] = fabs(A[r][c]); fin_in = fin_in && isf(A[r][c]); }
+    const double smax = max_tree<N * N>(aa);
+    if (!fin_in || smax == 0.0) return false;
     const double pivtol = 1e-15 * smax;
 #pragma unroll
     for (int k = 0; k < N; ++k) {
-        // partial pivoting: select-based swap keeps the matrix in registers
+        // partial pivoting, the first row of the largest |A_rk| (entries stay finite:
+        // |f| <= 1 below); a pairwise tree for four candidates, the sequential scan
+        // otherwise; select-based swap keeps the matrix in registers
         double amax = fabs(A[k][k]);
         int p = k;
+        if constexpr (N == 4) {
+            if (k == 0) {
+                const double a0 = fabs(A[0][0]), a1 = fabs(A[1][0]), a2 = fabs(A[2][0]), a3 = fabs(A[3][0]);
+                const bool s1 = a1 > a0, s3 = a3 > a2;
+                const double m01 = s1 ? a1 : a0, m23 = s3 ? a3 : a2;
+                const bool hi = m23 > m01;
+                amax = hi ? m23 : m01;
+                p = hi ? (s3 ? 3 : 2) : (s1 ? 1 : 0);
+            } else {
 #pragma unroll
-        for (int r = k + 1; r < N; ++r) if (fabs(A[r][k]) > amax) { amax = fabs(A[r][k]); p = r; }
+                for (int r = k + 1; r < N; ++r) if (fabs(A[r][k]) > amax) { amax = fabs(A[r][k]); p = r; }
+            }
+        } else {
+#pragma unroll
+            for (int r = k + 1; r < N; ++r) if (fabs(A[r][k]) > amax) { amax = fabs(A[r][k]); p = r; }
+        }
         if (amax < pivtol) return false;
 #pragma unroll
         for (int r = k + 1; r < N; ++r) {
@@ -724,9 +762,10 @@ __device__ __forceinline__ bool newton_linesearch(NewtonState& s, const CellIn& 
             const double Tn = jmax(s.Tr, T_MIN);
             if (Tn != s.tp.T) s.tp = eos_tpart<EK>(Tn, q);
             if (evalF<EK, NCH>(s.F, t[0], t[1], t[2], t[3], s.Tr, s.tp, c, q)) {
-                double trial = 0.0;
+                double tr[4 - rlo];       // evalF returned finite rows
 #pragma unroll
-                for (int r = rlo; r < 4; ++r) trial = jmax(trial, fabs(s.F[r] * c.rs[r]));
+                for (int r = rlo; r < 4; ++r) tr[r - rlo] = fabs(s.F[r] * c.rs[r]);
+                const double trial = max_tree<4 - rlo>(tr);
                 if (trial < nrm * (1 - 1e-4 * alpha)) return true;
             }
             alpha *= 0.5;
@@ -743,8 +782,15 @@ __device__ __forceinline__ bool newton_linesearch(NewtonState& s, const CellIn& 
         const int at = ahead ? (l - 8) >> 3 : 0;   // the trial whose Jacobian lane l evaluates
         const int ls = ahead ? at : 8 * round + l;
         const bool act = !ahead && ls < 12;
-        double alpha = alpha0;
-        for (int j = 0; j < ls; ++j) alpha *= 0.5;
+        // alpha0 halved ls times: one multiplication by 2^-ls, exact while the product
+        // is a normal number (alpha0 >= 2^-1000); the halving loop otherwise
+        double alpha;
+        if (alpha0 >= 0x1p-1000) {
+            alpha = alpha0 * __longlong_as_double((long long)(1023 - ls) << 52);
+        } else {
+            alpha = alpha0;
+            for (int j = 0; j < ls; ++j) alpha *= 0.5;
+        }
         double t[4];
         trial_point(alpha, t);
         double y[4] = {t[0], t[1], t[2], t[3]}, h = 1.0;
@@ -753,7 +799,12 @@ __device__ __forceinline__ bool newton_linesearch(NewtonState& s, const CellIn& 
         if (ahead) fd_point<NCH>(t, (l - 8) & 7, y, h, fdact, si);
         const bool fd0 = fdact && si == 0;
         const int slot = G >= 16 ? tcache_slot(s.tc, y[0]) : -1;
-        const double Tl = slot == 0 ? s.tc.T[0] : (slot == 1 ? s.tc.T[1] : exp(y[0]));
+        // exp only where a lane evaluates and its memo misses (an idle lane's T is
+        // never picked: the source lane below is an evaluating one)
+        double Tl = s.Tr;
+        if (slot == 0) Tl = s.tc.T[0];
+        else if (slot == 1) Tl = s.tc.T[1];
+        else if (act || fdact) Tl = exp(y[0]);
         const double Tn = jmax(Tl, T_MIN);
         TPart tl = s.tp;
         double Fl[4] = {0.0, 0.0, 0.0, 0.0};
@@ -767,10 +818,10 @@ __device__ __forceinline__ bool newton_linesearch(NewtonState& s, const CellIn& 
             }
             okl = evalF<EK, NCH>(Fl, y[0], y[1], y[2], y[3], Tl, tl, c, q);
             if (act && okl) {
-                double trial = 0.0;
+                double tr[4 - rlo];       // okl: finite rows
 #pragma unroll
-                for (int r = rlo; r < 4; ++r) trial = jmax(trial, fabs(Fl[r] * c.rs[r]));
-                accl = trial < nrm * (1 - 1e-4 * alpha);
+                for (int r = rlo; r < 4; ++r) tr[r - rlo] = fabs(Fl[r] * c.rs[r]);
+                accl = max_tree<4 - rlo>(tr) < nrm * (1 - 1e-4 * alpha);
             }
         }
         const unsigned b = gbits<G>(accl);
@@ -794,19 +845,27 @@ __device__ __forceinline__ bool newton_linesearch(NewtonState& s, const CellIn& 
     return false;
 }
 
+constexpr double NEWTON_TOL = 1e-12;
+// max_r |F_r| s_r over the rows of the system (src2d/primrec2d.jl:594-601)
+template <bool NCH>
+__device__ __forceinline__ double newton_scaled_norm(const NewtonState& s, const CellIn& c) {
+    constexpr int rlo = NCH ? 0 : 1;
+    double sr[4 - rlo];
+#pragma unroll
+    for (int r = rlo; r < 4; ++r) sr[r - rlo] = s.F[r] * c.rs[r];
+    return max_abs_nan<4 - rlo>(sr);
+}
+
 // Newton iterations it .. itcap of _cons_to_prim_attempt_2d! (src2d/primrec2d.jl:591-692)
 template <int EK, bool NCH, int G>
 __device__ __forceinline__ void newton_iterate(NewtonState& s, const CellIn& c, const EosP& q, int itcap) {
     constexpr int NU = NCH ? 4 : 3;
     constexpr int rlo = NCH ? 0 : 1;
-    constexpr double tol = 1e-12;
     s.jnext = false;
     for (; s.it <= NEWTON_MAXIT; ++s.it) {
         if (s.it > itcap) return;                    // paused before iteration s.it
-        double nrm = 0.0;
-#pragma unroll
-        for (int r = rlo; r < 4; ++r) nrm = jmax(nrm, fabs(s.F[r] * c.rs[r]));
-        if (nrm < tol) { s.converged = true; s.reason = PRR_CONVERGED; s.iters = s.it - 1; return; }
+        const double nrm = newton_scaled_norm<NCH>(s, c);
+        if (nrm < NEWTON_TOL) { s.converged = true; s.reason = PRR_CONVERGED; s.iters = s.it - 1; return; }
         double J[4][4];
         bool ahead = false;
         if constexpr (G >= 16) {
@@ -831,9 +890,13 @@ __device__ __forceinline__ void newton_iterate(NewtonState& s, const CellIn& c, 
             delta[r] = 0.0;
         }
         if (!solve_gauss<NU>(J, Fact, delta)) { s.reason = PRR_LINSOLVE_FAILED; return; }
-        double alpha = 1.0;
+        // alpha = min(1, min over |delta_k| > 1 of 1/|delta_k|) = 1/max|delta_k| when that
+        // exceeds 1 (delta is finite, and 1/x rounds monotonically)
+        double ad[NU];
 #pragma unroll
-        for (int k = 0; k < NU; ++k) { double ad = fabs(delta[k]); if (ad > 1.0) alpha = jmin(alpha, 1.0 / ad); }
+        for (int k = 0; k < NU; ++k) ad[k] = fabs(delta[k]);
+        const double admax = max_tree<NU>(ad);
+        const double alpha = admax > 1.0 ? 1.0 / admax : 1.0;
         if (!newton_linesearch<EK, NCH, G>(s, c, q, delta, alpha, nrm)) {
             s.reason = PRR_LINESEARCH_FAILED;
             return;
