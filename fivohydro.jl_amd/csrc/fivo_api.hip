@@ -212,12 +212,97 @@ int stage(Ctx& c, double* Ub, const double* Uold, double* Uout, double tau, doub
     return 0;
 }
 
-// step_ssprk2_2d! / step_ssprk3_2d! (src2d/timestepper2d.jl:74-188)
+// One RK stage's launches up to its admissibility count, without reading it back:
+// the near-bound cells' inversion test always runs (a no-op when no cell is pending).
+int stage_launch(Ctx& c, double* Ub, const double* Uold, double* Uout, double tau, double tau_state, double D,
+                 int combine, double wold, double wnew, int bc, bool bc_first) {
+    int rc;
+    if ((rc = reset_scalars(c)) < 0) return rc;
+    if (bc_first && (rc = do_bc(c, Ub, bc)) < 0) return rc;
+    FaceLaunch f{};
+    f.Ub = Ub; f.Uold = Uold; f.Uout = Uout; f.dUout = nullptr; f.mode = 1; f.combine = combine;
+    f.tau = tau; f.tau_state = tau_state; f.D = D; f.wold = wold; f.wnew = wnew; f.force = nullptr;
+    if ((rc = do_prim_faces(c, Ub, tau, f)) < 0) return rc;
+    if ((rc = do_bc(c, Uout, bc)) < 0) return rc;
+    HIPCHK(&c, launch_admissible_pending(c, Uout, tau_state));
+    c.nrhs += 1;
+    return FIVO_OK;
+}
+
+// read_scalars' host state from a copy of the device scalars
+void take_scalars(Ctx& c, const DevScalars& s) {
+    *c.sc_host = s;
+    c.amax_host = __builtin_bit_cast(double, (unsigned long long)s.amax_bits);
+    c.last_primfail = (int64_t)s.nprimfail;
+    c.last_vacuum = (int64_t)s.nvacuum;
+}
+
+// Every stage of one SSPRK attempt (no MOOD retries) launched back to back, the
+// device scalars copied to a pinned slot after each stage, one synchronisation for
+// the attempt instead of one per stage. *failed = the first stage whose output is
+// inadmissible (1-based), 0 if none. After a failure the later stages' changes that
+// the retry would see are rolled back -- the Newton warm starts (copied after each
+// stage but the last) and the device counters (from the failed stage's slot) -- so
+// the halved retry starts from exactly what the stage-by-stage loop leaves.
+int step_all_stages(Ctx& c, int integ, double tau, double D, int bc, int* failed) {
+    const int ns = integ == FIVO_SSPRK3 ? 3 : 2;
+    double* U = c.U[c.cur];
+    double* S1 = c.U[(c.cur + 1) % 4];
+    double* S2 = c.U[(c.cur + 2) % 4];
+    double* S3 = c.U[(c.cur + 3) % 4];
+    const int64_t nrhs0 = c.nrhs;
+    const size_t wsn = 4 * (size_t)c.p.pitch;   // x0_yT, x0_phi, x0_ux, x0_uy: consecutive pitch blocks
+    for (int k = 0; k < ns; ++k) {
+        int rc;
+        if (k == 0) rc = stage_launch(c, U, U, S1, tau, tau + D, D, 0, 1.0, 0.0, bc, true);
+        else if (integ == FIVO_SSPRK3 && k == 1) rc = stage_launch(c, S1, U, S2, tau + D, tau + D, D, 1, 0.75, 0.25, bc, false);
+        else if (integ == FIVO_SSPRK3) rc = stage_launch(c, S2, U, S3, tau + 0.5 * D, tau + D, D, 2, 0.0, 0.0, bc, false);
+        else rc = stage_launch(c, S1, U, S2, tau + D, tau + D, D, 1, 0.5, 0.5, bc, false);
+        if (rc < 0) return rc;
+        HIPCHK(&c, hipMemcpyAsync(&c.sc_stage_host[k], c.sc, sizeof(DevScalars), hipMemcpyDeviceToHost, c.stream));
+        if (k + 1 < ns)
+            HIPCHK(&c, hipMemcpyAsync(c.ws + k * wsn, c.w.x0_yT, sizeof(double) * wsn, hipMemcpyDeviceToDevice,
+                                      c.stream));
+    }
+    HIPCHK(&c, hipStreamSynchronize(c.stream));
+    resolve_timing(c);
+    *failed = 0;
+    for (int k = 0; k < ns; ++k)
+        if (c.sc_stage_host[k].nbad != 0) { *failed = k + 1; break; }
+    const int last = *failed ? *failed - 1 : ns - 1;
+    if (*failed && *failed < ns) {
+        HIPCHK(&c, hipMemcpyAsync(c.w.x0_yT, c.ws + last * wsn, sizeof(double) * wsn, hipMemcpyDeviceToDevice,
+                                  c.stream));
+        HIPCHK(&c, hipMemcpyAsync(c.sc, &c.sc_stage_host[last], sizeof(DevScalars), hipMemcpyHostToDevice, c.stream));
+        c.nrhs = nrhs0 + *failed;
+    }
+    take_scalars(c, c.sc_stage_host[last]);
+    return FIVO_OK;
+}
+
+// step_ssprk2_2d! / step_ssprk3_2d! (src2d/timestepper2d.jl:74-188). Without MOOD
+// retries every attempt but the last launches its stages back to back
+// (step_all_stages); the last attempt, and every attempt with MOOD, checks each
+// stage before the next, so a step that fails leaves the state of the reference's
+// loop behind.
 int step_dev(Ctx& c, int integ, double tau, double dtau, int bc, int max_halve, int max_mood, int* ok,
              double* used) {
     double D = dtau;
     int rc;
+    const bool batch = integ == FIVO_SSPRK3 || max_mood == 0;
     for (int h = 0; h <= max_halve; ++h) {
+        if (batch && h < max_halve) {
+            int failed = 0;
+            if ((rc = step_all_stages(c, integ, tau, D, bc, &failed)) < 0) return rc;
+            if (!failed) {
+                c.cur = (c.cur + (integ == FIVO_SSPRK3 ? 3 : 2)) % 4;
+                *ok = 1; *used = D;
+                return FIVO_OK;
+            }
+            D *= 0.5;
+            c.nhalvings += 1;
+            continue;
+        }
         double* U = c.U[c.cur];
         double* S1 = c.U[(c.cur + 1) % 4];
         double* S2 = c.U[(c.cur + 2) % 4];
@@ -499,7 +584,7 @@ int fivo_create(const fivo_grid_desc* g, const fivo_model_desc* m, fivo_ctx** ou
         delete c; return fail(nullptr, FIVO_ERR_HIP, hipGetErrorString(e));
     }
     const size_t P = p.pitch;
-    const size_t ndbl = 4 * nv * P + nv * P + nv * P + 24 * P + 3 * (size_t)c->npartials + 64;
+    const size_t ndbl = 4 * nv * P + nv * P + nv * P + 8 * P + 24 * P + 3 * (size_t)c->npartials + 64;
     c->ntiles_max = ((p.Nx + 6) / 7) * ((p.Ny + 7) / 8);   // the smallest face tile, 7 x 8
     const size_t nbyte = ndbl * sizeof(double) + 4 * P + sizeof(PrimJob) * P + (size_t)c->ntiles_max + 256;
     c->pool_bytes = nbyte;
@@ -511,6 +596,7 @@ int fivo_create(const fivo_grid_desc* g, const fivo_model_desc* m, fivo_ctx** ou
     for (int k = 0; k < 4; ++k) { c->U[k] = d; d += nv * P; }
     c->dU = d; d += nv * P;
     c->snap = d; d += nv * P;
+    c->ws = d; d += 8 * P;
     double** wd[] = {&c->w.yT, &c->w.phi, &c->w.mu, &c->w.alpha, &c->w.ux, &c->w.uy, &c->w.P, &c->w.n, &c->w.e,
                      &c->w.vx, &c->w.vy, &c->w.x0_yT, &c->w.x0_phi, &c->w.x0_ux, &c->w.x0_uy, &c->w.ux_prev,
                      &c->w.uy_prev, &c->w.alpha_prev, &c->w.T_prev, &c->w.nux_prev, &c->w.nuy_prev,
@@ -527,6 +613,9 @@ int fivo_create(const fivo_grid_desc* g, const fivo_model_desc* m, fivo_ctx** ou
     c->dirty = b; b += c->ntiles_max;
     if ((e = hipMalloc(&c->sc, sizeof(DevScalars))) != hipSuccess ||
         (e = hipHostMalloc(&c->sc_host, sizeof(DevScalars), hipHostMallocDefault)) != hipSuccess ||
+        (e = hipHostMalloc(&c->sc_stage_host, 4 * sizeof(DevScalars), hipHostMallocDefault)) != hipSuccess ||
+        (e = hipHostMalloc(&c->partials_host, sizeof(double) * 2 * (size_t)c->npartials, hipHostMallocDefault)) !=
+            hipSuccess ||
         (e = hipEventCreate(&c->ev[0])) != hipSuccess || (e = hipEventCreate(&c->ev[1])) != hipSuccess ||
         (e = hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking)) != hipSuccess ||
         (e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming)) != hipSuccess ||
@@ -554,6 +643,8 @@ void fivo_destroy(fivo_ctx* ctx) {
     if (c->pool) (void)hipFree(c->pool);
     if (c->sc) (void)hipFree(c->sc);
     if (c->sc_host) (void)hipHostFree(c->sc_host);
+    if (c->sc_stage_host) (void)hipHostFree(c->sc_stage_host);
+    if (c->partials_host) (void)hipHostFree(c->partials_host);
     for (auto& ev : c->ev) if (ev) (void)hipEventDestroy(ev);
     for (auto& ev : c->evpool) (void)hipEventDestroy(ev);
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
@@ -841,6 +932,23 @@ int fivo_run(fivo_ctx* ctx, const fivo_run_opts* o, fivo_run_result* r) {
     int64_t nsteps = 0, nprimfail = 0, nvacuum = 0;
     if (o->on_dump) { o->on_dump(o->user, tau, ctx); next_dump = o->tau0 + o->dump_dt; }
     const bool dissip = c.md.enable_shear || c.md.enable_bulk || c.md.enable_diff;
+    // The refresh pass's counts and the shear projection's residual of a step are
+    // copied to pinned memory without a synchronisation and taken in at the next one
+    // (the next step's, a dump's or the run's end): the step's only wait is its stages'.
+    bool defer_refresh = false, defer_shear = false;
+    auto take_deferred = [&]() {
+        if (defer_refresh) {
+            nprimfail += (int64_t)c.sc_stage_host[3].nprimfail;
+            nvacuum += (int64_t)c.sc_stage_host[3].nvacuum;
+            defer_refresh = false;
+        }
+        if (defer_shear) {
+            double m = 0.0;
+            for (int b = 0; b < c.npartials; ++b) m = c.partials_host[2 * b] > m ? c.partials_host[2 * b] : m;
+            max_shear = m > max_shear ? m : max_shear;
+            defer_shear = false;
+        }
+    };
     while (tau < o->taufinal && nsteps < o->max_steps) {
         double D;
         fivo_compute_dt(ctx, tau, o->cfl, o->cfltau, &D);
@@ -852,6 +960,7 @@ int fivo_run(fivo_ctx* ctx, const fivo_run_opts* o, fivo_run_result* r) {
         double used = 0;
         rc = step_dev(c, o->integrator, tau, D, o->bc, o->max_halve, o->max_mood, &ok, &used);
         if (rc < 0) return rc;
+        take_deferred();   // step_dev synchronised after the deferred copies
         if (!ok) {
             r->ok = 0; r->tau = tau; r->nsteps = nsteps; r->nprimfail = nprimfail; r->nvacuum = nvacuum;
             r->max_shear_res = max_shear;
@@ -868,23 +977,34 @@ int fivo_run(fivo_ctx* ctx, const fivo_run_opts* o, fivo_run_result* r) {
         tau += used;
         nsteps += 1;
         if (dissip) {
+            // update_primitives_2d! + relax_dissipative_2d!; amax stays the last RHS's
             if ((rc = reset_scalars(c)) < 0) return rc;
             if ((rc = do_bc(c, c.U[c.cur], o->bc)) < 0) return rc;
             if ((rc = do_prim(c, c.U[c.cur], tau)) < 0) return rc;
             if ((rc = relax_dev(c, tau, used)) < 0) return rc;
-            double amax = c.amax_host;
-            if ((rc = read_scalars(c)) < 0) return rc;
-            c.amax_host = amax;
+            HIPCHK(&c, hipMemcpyAsync(&c.sc_stage_host[3], c.sc, sizeof(DevScalars), hipMemcpyDeviceToHost,
+                                      c.stream));
+            defer_refresh = true;
+        } else {
+            nprimfail += c.last_primfail;
+            nvacuum += c.last_vacuum;
         }
-        nprimfail += c.last_primfail;
-        nvacuum += c.last_vacuum;
         if (c.md.enable_shear) {
-            double mx, mean;
-            if ((rc = project_shear_dev(c, &mx, &mean)) < 0) return rc;
-            max_shear = mx > max_shear ? mx : max_shear;
+            HIPCHK(&c, launch_project_shear(c, c.U[c.cur], c.md.shear_constraint_project ? 1 : 0));
+            HIPCHK(&c, hipMemcpyAsync(c.partials_host, c.partials, sizeof(double) * 2 * (size_t)c.npartials,
+                                      hipMemcpyDeviceToHost, c.stream));
+            defer_shear = true;
         }
-        if (o->on_dump && tau >= next_dump - 1e-12) { o->on_dump(o->user, tau, ctx); next_dump += o->dump_dt; }
+        if (o->on_dump && tau >= next_dump - 1e-12) {
+            HIPCHK(&c, hipStreamSynchronize(c.stream));
+            take_deferred();
+            o->on_dump(o->user, tau, ctx);
+            next_dump += o->dump_dt;
+        }
     }
+    HIPCHK(&c, hipStreamSynchronize(c.stream));
+    resolve_timing(c);
+    take_deferred();
     if (o->on_dump) o->on_dump(o->user, tau, ctx);
     double Qout, maxu, minp;
     if ((rc = diag(c, 1, &Qout, &maxu, &minp)) < 0) return rc;

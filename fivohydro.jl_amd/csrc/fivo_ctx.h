@@ -105,6 +105,13 @@ struct Ctx {
     DevWork w;
     DevScalars* sc = nullptr;       // device
     DevScalars* sc_host = nullptr;  // pinned mirror
+    // a step whose stages are all launched before any is checked: the scalars after
+    // each stage (slots 0-2) and after the refresh pass (slot 3), pinned; the Newton
+    // warm starts after stages 1 and 2 (2 x 4 pitch doubles, the rollback of a failed
+    // stage); the shear projection's partials, pinned, read at the next synchronisation
+    DevScalars* sc_stage_host = nullptr;
+    double* ws = nullptr;
+    double* partials_host = nullptr;
     double amax_host = 0.0;         // last RHS amax, as read back
     std::string err;
     // tile shape of the fused face kernel (chosen by grid size or autotuned)
