@@ -104,10 +104,14 @@ void resolve_timing(Ctx& c) {
     c.ev_used = 0;
 }
 
+// zero the device scalars first..last (fields of DevScalars, with the lines between)
+hipError_t clear_scalars(Ctx& c, unsigned long long* first, unsigned long long* last) {
+    return hipMemsetAsync(first, 0, (size_t)((char*)(last + 1) - (char*)first), c.stream);
+}
 int reset_scalars(Ctx& c) {
     // amax, nbad, nprimfail, nvacuum, npending, ncold -> 0 (newton_iters kept)
-    HIPCHK(&c, hipMemsetAsync(c.sc, 0, 4 * sizeof(unsigned long long), c.stream));
-    HIPCHK(&c, hipMemsetAsync(&c.sc->npending, 0, 2 * sizeof(unsigned long long), c.stream));
+    HIPCHK(&c, clear_scalars(c, &c.sc->amax_bits, &c.sc->nvacuum));
+    HIPCHK(&c, clear_scalars(c, &c.sc->npending, &c.sc->ncold));
     return FIVO_OK;
 }
 int read_scalars(Ctx& c) {
@@ -796,7 +800,7 @@ int fivo_rhs(fivo_ctx* ctx, double tau, int bc, const uint8_t* force, double* dU
         fdev = c.w.force;
     }
     // this call's retries and staged fallbacks
-    HIPCHK(&c, hipMemsetAsync(&c.sc->nretry, 0, 2 * sizeof(unsigned long long), c.stream));
+    HIPCHK(&c, clear_scalars(c, &c.sc->nretry, &c.sc->nstaged));
     int rc = rhs_dev(c, tau, bc, fdev);
     if (rc < 0) return rc;
     if (dU) {
@@ -816,7 +820,7 @@ int fivo_update_primitives(fivo_ctx* ctx, double tau, int bc, fivo_counts* cnt) 
     Ctx& c = *reinterpret_cast<Ctx*>(ctx);
     int rc;
     if ((rc = reset_scalars(c)) < 0) return rc;
-    HIPCHK(&c, hipMemsetAsync(&c.sc->nretry, 0, 2 * sizeof(unsigned long long), c.stream));
+    HIPCHK(&c, clear_scalars(c, &c.sc->nretry, &c.sc->nstaged));
     // keep amax: update_primitives_2d! does not touch the signal speeds
     if ((rc = do_bc(c, c.U[c.cur], bc)) < 0) return rc;
     if ((rc = do_prim(c, c.U[c.cur], tau)) < 0) return rc;
@@ -836,7 +840,7 @@ int fivo_background_slice(fivo_ctx* ctx, int32_t k, int32_t nt, double* T, doubl
     if (!T || !vx || !vy || nt < 1 || k < 0 || k >= nt)
         return fail(&c, FIVO_ERR_ARG, "background_slice: need T, vx, vy and 0 <= k < nt");
     const int64_t cells = (int64_t)c.p.Nx * c.p.Ny;
-    HIPCHK(&c, hipMemsetAsync(&c.sc->bg_umax_bits, 0, 2 * sizeof(unsigned long long), c.stream));
+    HIPCHK(&c, clear_scalars(c, &c.sc->bg_umax_bits, &c.sc->bg_vmax_bits));
     if (device) {
         Timer t(c, TK_BG);
         HIPCHK(&c, launch_background(c, cells * k, T, vx, vy, n));

@@ -157,14 +157,22 @@ __device__ __forceinline__ void rare_counts(DevScalars* sc, unsigned long long n
     }
 }
 
+// The pass's counts summed over the block (up to four wavefronts) and added with
+// one atomic per nonzero sum: thousands of wavefronts adding to one counter are
+// serialised on its line. Every thread of the block calls it.
 __device__ __forceinline__ void prim_counts(DevScalars* sc, unsigned long long its, unsigned long long nfail,
                                             unsigned long long nvac, unsigned long long nq = 0) {
+    __shared__ unsigned long long part[4][4];
     its = wave_sum(its); nfail = wave_sum(nfail); nvac = wave_sum(nvac); nq = wave_sum(nq);
-    if ((threadIdx.x & 63) == 0) {
-        if (its) atomicAdd(&sc->newton_iters, its);
-        if (nfail) atomicAdd(&sc->nprimfail, nfail);
-        if (nvac) atomicAdd(&sc->nvacuum, nvac);
-        if (nq) atomicAdd(&sc->nqueued, nq);
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { part[wv][0] = its; part[wv][1] = nfail; part[wv][2] = nvac; part[wv][3] = nq; }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        const int nw = (blockDim.x + 63) >> 6;
+        unsigned long long v = 0;
+        for (int k = 0; k < nw; ++k) v += part[k][threadIdx.x];
+        unsigned long long* dst[4] = {&sc->newton_iters, &sc->nprimfail, &sc->nvacuum, &sc->nqueued};
+        if (v) atomicAdd(dst[threadIdx.x], v);
     }
 }
 

@@ -44,24 +44,29 @@ struct Params {
     double kappa_coeff, transport_mass, tauN_coeff, deltaN_factor, nu_clip_factor, vacuum_n_lo, vacuum_n_hi;
 };
 
-// Small device-resident scalars read back once per stage
+// Small device-resident scalars read back once per stage. Each on a 128-byte line
+// of its own: device-wide atomics on one line are serialised (~12 ns apiece on
+// MI355X), so counters sharing a line would also wait for each other's updates.
+#define FIVO_SCALAR(name) unsigned long long name; unsigned long long name##_line_[15]
 struct DevScalars {
-    unsigned long long amax_bits;   // max |signal speed| of the last RHS (positive double bits)
-    unsigned long long nbad;        // inadmissible cells of the last stage
-    unsigned long long nprimfail;   // last primitive pass
-    unsigned long long nvacuum;
-    unsigned long long newton_iters;// cumulative
-    unsigned long long shear_max_bits;
-    unsigned long long npending;    // near-bound cells awaiting the inversion test
-    unsigned long long ncold;       // cells of the current primitive pass queued for the cooperative solve
-    unsigned long long nlearned;    // cumulative: cells started from the learned seed
-    unsigned long long nfallback;   // cumulative: of those, cells that fell back to the cold seed
-    unsigned long long nqueued;     // cumulative: cells queued for the cooperative kernel
-    unsigned long long bg_umax_bits;// background slice: max |u|, max |v| (double bits; NaN sticks)
-    unsigned long long bg_vmax_bits;
-    unsigned long long nretry;      // cumulative: warm-started solves that failed and were retried from the cold seed
-    unsigned long long nstaged;     // cumulative: solves closed by the staged phi fallback (bisection of the charge row)
+    FIVO_SCALAR(amax_bits);     // max |signal speed| of the last RHS (positive double bits)
+    FIVO_SCALAR(nbad);          // inadmissible cells of the last stage
+    FIVO_SCALAR(nprimfail);     // last primitive pass
+    FIVO_SCALAR(nvacuum);
+    FIVO_SCALAR(newton_iters);  // cumulative
+    FIVO_SCALAR(shear_max_bits);
+    FIVO_SCALAR(npending);      // near-bound cells awaiting the inversion test
+    FIVO_SCALAR(ncold);         // cells of the current primitive pass queued for the cooperative solve
+    FIVO_SCALAR(nlearned);      // cumulative: cells started from the learned seed
+    FIVO_SCALAR(nfallback);     // cumulative: of those, cells that fell back to the cold seed
+    FIVO_SCALAR(nqueued);       // cumulative: cells queued for the cooperative kernel
+    FIVO_SCALAR(bg_umax_bits);  // background slice: max |u|, max |v| (double bits; NaN sticks)
+    FIVO_SCALAR(bg_vmax_bits);
+    FIVO_SCALAR(nretry);        // cumulative: warm-started solves that failed and were retried from the cold seed
+    FIVO_SCALAR(nstaged);       // cumulative: solves closed by the staged phi fallback (bisection of the charge row)
 };
+#undef FIVO_SCALAR
+
 
 // Per-cell work arrays on the device (Work2D, src2d/work2d.jl:15-100)
 // a cell queued by the warm primitive kernel: resume its Newton at iteration

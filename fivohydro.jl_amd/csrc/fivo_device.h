@@ -76,6 +76,11 @@ __device__ __forceinline__ double max_abs_nan(const double* v) {
     return isnan_(s) ? s : m;
 }
 __device__ __forceinline__ double dinf() { return __longlong_as_double(0x7ff0000000000000LL); }
+// Julia's max(x, c) and clamp(x, a, b) where x is known not to be NaN (Newton
+// iterates and trial points are finite, their temperatures exp of a finite yT):
+// one v_max / v_min each, the same values
+__device__ __forceinline__ double max_nn(double x, double c) { return fmax(x, c); }
+__device__ __forceinline__ double clamp_nn(double x, double a, double b) { return fmin(fmax(x, a), b); }
 
 // mc_limiter (src/utils.jl:14-22)
 __device__ __forceinline__ double mc_limiter(double a, double b) {
@@ -644,7 +649,7 @@ __device__ __forceinline__ void fd_point(const double x[4], int l, double y[4], 
     act = k < NU;
     si = NCH ? k : (k == 0 ? 0 : k + 1);
     const double xs = si == 0 ? x[0] : si == 1 ? x[1] : si == 2 ? x[2] : x[3];
-    h = 1e-6 * jmax(1.0, fabs(xs));
+    h = 1e-6 * max_nn(fabs(xs), 1.0);
     const double xv = (l & 1) ? xs - h : xs + h;
     y[0] = si == 0 ? xv : x[0];
     y[1] = si == 1 ? xv : x[1];
@@ -671,15 +676,15 @@ __device__ __forceinline__ bool newton_jacobian(NewtonState& s, const CellIn& c,
         for (int k = 0; k < NU; ++k) {
             const int si = NCH ? k : (k == 0 ? 0 : k + 1);
             double xs = s.x[si];
-            double h = 1e-6 * jmax(1.0, fabs(xs));
+            double h = 1e-6 * max_nn(fabs(xs), 1.0);
             double Fp[4], Fm[4];
             double xp[4] = {s.x[0], s.x[1], s.x[2], s.x[3]}, xm[4] = {s.x[0], s.x[1], s.x[2], s.x[3]};
             xp[si] = xs + h; xm[si] = xs - h;
             bool a, b;
             if (si == 0) {
                 const double Tp = exp(xp[0]), Tm = exp(xm[0]);
-                TPart tpp = eos_tpart<EK>(jmax(Tp, T_MIN), q);
-                TPart tpm = eos_tpart<EK>(jmax(Tm, T_MIN), q);
+                TPart tpp = eos_tpart<EK>(max_nn(Tp, T_MIN), q);
+                TPart tpm = eos_tpart<EK>(max_nn(Tm, T_MIN), q);
                 a = evalF<EK, NCH>(Fp, xp[0], xp[1], xp[2], xp[3], Tp, tpp, c, q);
                 b = evalF<EK, NCH>(Fm, xm[0], xm[1], xm[2], xm[3], Tm, tpm, c, q);
             } else {
@@ -710,7 +715,7 @@ __device__ __forceinline__ bool newton_jacobian(NewtonState& s, const CellIn& c,
             tl = slot == 0 ? s.tc.tp[0] : s.tc.tp[1];
         } else {
             Tl = exp(y[0]);
-            tl = eos_tpart<EK>(jmax(Tl, T_MIN), q);
+            tl = eos_tpart<EK>(max_nn(Tl, T_MIN), q);
             if (G >= 16) tcache_put(s.tc, y[0], Tl, tl);
         }
     }
@@ -747,10 +752,11 @@ __device__ __forceinline__ bool newton_linesearch(NewtonState& s, const CellIn& 
             const int si = NCH ? k : (k == 0 ? 0 : k + 1);
             t[si] += alpha * delta[k];
         }
-        t[0] = jclamp(t[0], yT_lo, yT_hi);
-        if (NCH) t[1] = jclamp(t[1], -PHI_CAP, PHI_CAP);
-        t[2] = jclamp(t[2], -U_CAP, U_CAP);
-        t[3] = jclamp(t[3], -U_CAP, U_CAP);
+        // finite: x and delta are, and |alpha delta_k| <= 1
+        t[0] = clamp_nn(t[0], yT_lo, yT_hi);
+        if (NCH) t[1] = clamp_nn(t[1], -PHI_CAP, PHI_CAP);
+        t[2] = clamp_nn(t[2], -U_CAP, U_CAP);
+        t[3] = clamp_nn(t[3], -U_CAP, U_CAP);
     };
     if constexpr (G == 0) {
         double alpha = alpha0;
@@ -759,7 +765,7 @@ __device__ __forceinline__ bool newton_linesearch(NewtonState& s, const CellIn& 
             trial_point(alpha, t);
             s.x[0] = t[0]; s.x[1] = t[1]; s.x[2] = t[2]; s.x[3] = t[3];
             s.Tr = exp(t[0]);
-            const double Tn = jmax(s.Tr, T_MIN);
+            const double Tn = max_nn(s.Tr, T_MIN);
             if (Tn != s.tp.T) s.tp = eos_tpart<EK>(Tn, q);
             if (evalF<EK, NCH>(s.F, t[0], t[1], t[2], t[3], s.Tr, s.tp, c, q)) {
                 double tr[4 - rlo];       // evalF returned finite rows
@@ -805,7 +811,7 @@ __device__ __forceinline__ bool newton_linesearch(NewtonState& s, const CellIn& 
         if (slot == 0) Tl = s.tc.T[0];
         else if (slot == 1) Tl = s.tc.T[1];
         else if (act || fdact) Tl = exp(y[0]);
-        const double Tn = jmax(Tl, T_MIN);
+        const double Tn = max_nn(Tl, T_MIN);
         TPart tl = s.tp;
         double Fl[4] = {0.0, 0.0, 0.0, 0.0};
         bool accl = false, okl = true;

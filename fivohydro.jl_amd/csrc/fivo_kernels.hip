@@ -252,18 +252,6 @@ hipError_t launch_prim_seed(Ctx& c, const double* U, double tau) {
                                        c.stream, U, p, tau, c.w, c.sc, c.jobs, c.mlp));
     return hipGetLastError();
 }
-template <int EK, bool NCH, int G>
-static hipError_t coop_launch(int nb, int lds, bool& lds_set, hipStream_t st, const double* U, const Params& p,
-                              double tau, Ctx& c) {
-    if (!lds_set) {
-        const hipError_t e = hipFuncSetAttribute((const void*)k_prim_coop<EK, NCH, G>,
-                                                 hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-        if (e != hipSuccess) return e;
-        lds_set = true;
-    }
-    hipLaunchKernelGGL((k_prim_coop<EK, NCH, G>), dim3(nb), dim3(256), lds, st, U, p, tau, c.w, c.sc, c.jobs);
-    return hipGetLastError();
-}
 hipError_t launch_prim_coop(Ctx& c, const double* U, double tau) {
     const Params& p = c.p;
     // one group of G lanes per queued cell. The queue of a warm pass holds ~0.5 %
@@ -272,21 +260,18 @@ hipError_t launch_prim_coop(Ctx& c, const double* U, double tau) {
     // are fastest; larger grids queue enough cells to fill the GPU, where fewer
     // lanes per cell keep more cells in flight. 2048 blocks hold the queue of a
     // warm pass and stride over the queue of a cold one.
-    // The kernel runs one wavefront per SIMD (its registers); its blocks also reserve
-    // COOP_LDS bytes of LDS (unused) so that no face-tile block, which runs beside
-    // it on a second stream, shares a CU with it and slows the latency-bound solves.
     const int G = p.Ntot <= (1 << 19) ? 32 : (p.Ntot <= (1 << 21) ? 16 : 8);
     const int nb = std::min(nblk((int64_t)p.Ntot * G, 256), 2048);
-    constexpr int COOP_LDS = 112 * 1024;
-    static bool lds_set[3] = {false, false, false};
-    hipError_t e = hipSuccess;
     if (G == 32)
-        EOS_DISPATCH(c, (e = coop_launch<EK, NCH, 32>(nb, COOP_LDS, lds_set[0], c.stream, U, p, tau, c)));
+        EOS_DISPATCH(c, hipLaunchKernelGGL((k_prim_coop<EK, NCH, 32>), dim3(nb), dim3(256), 0, c.stream, U, p, tau,
+                                           c.w, c.sc, c.jobs));
     else if (G == 16)
-        EOS_DISPATCH(c, (e = coop_launch<EK, NCH, 16>(nb, COOP_LDS, lds_set[1], c.stream, U, p, tau, c)));
+        EOS_DISPATCH(c, hipLaunchKernelGGL((k_prim_coop<EK, NCH, 16>), dim3(nb), dim3(256), 0, c.stream, U, p, tau,
+                                           c.w, c.sc, c.jobs));
     else
-        EOS_DISPATCH(c, (e = coop_launch<EK, NCH, 8>(nb, COOP_LDS, lds_set[2], c.stream, U, p, tau, c)));
-    return e != hipSuccess ? e : hipGetLastError();
+        EOS_DISPATCH(c, hipLaunchKernelGGL((k_prim_coop<EK, NCH, 8>), dim3(nb), dim3(256), 0, c.stream, U, p, tau,
+                                           c.w, c.sc, c.jobs));
+    return hipGetLastError();
 }
 hipError_t launch_prim(Ctx& c, const double* U, double tau) {
     hipError_t e = launch_prim_warm(c, U, tau);
