@@ -109,9 +109,9 @@ hipError_t clear_scalars(Ctx& c, unsigned long long* first, unsigned long long* 
     return hipMemsetAsync(first, 0, (size_t)((char*)(last + 1) - (char*)first), c.stream);
 }
 int reset_scalars(Ctx& c) {
-    // amax, nbad, nprimfail, nvacuum, npending, ncold -> 0 (newton_iters kept)
+    // amax, nbad, nprimfail, nvacuum, npending, ncold, nwarm -> 0 (newton_iters kept)
     HIPCHK(&c, clear_scalars(c, &c.sc->amax_bits, &c.sc->nvacuum));
-    HIPCHK(&c, clear_scalars(c, &c.sc->npending, &c.sc->ncold));
+    HIPCHK(&c, clear_scalars(c, &c.sc->npending, &c.sc->nwarm));
     return FIVO_OK;
 }
 int read_scalars(Ctx& c) {
@@ -590,7 +590,7 @@ int fivo_create(const fivo_grid_desc* g, const fivo_model_desc* m, fivo_ctx** ou
     const size_t P = p.pitch;
     const size_t ndbl = 4 * nv * P + nv * P + nv * P + 8 * P + 24 * P + 3 * (size_t)c->npartials + 64;
     c->ntiles_max = ((p.Nx + 6) / 7) * ((p.Ny + 7) / 8);   // the smallest face tile, 7 x 8
-    const size_t nbyte = ndbl * sizeof(double) + 4 * P + sizeof(PrimJob) * P + (size_t)c->ntiles_max + 256;
+    const size_t nbyte = ndbl * sizeof(double) + 4 * P + sizeof(PrimJob) * P + sizeof(int) * P + (size_t)c->ntiles_max + 256;
     c->pool_bytes = nbyte;
     if ((e = hipMalloc(&c->pool, nbyte)) != hipSuccess) {
         (void)hipStreamDestroy(c->stream); delete c;
@@ -614,6 +614,7 @@ int fivo_create(const fivo_grid_desc* g, const fivo_model_desc* m, fivo_ctx** ou
     c->w.force = b; b += P;
     c->w.ftmp = b; b += P;
     c->jobs = reinterpret_cast<PrimJob*>(b); b += sizeof(PrimJob) * P;
+    c->wlist = reinterpret_cast<int*>(b); b += sizeof(int) * P;
     c->dirty = b; b += c->ntiles_max;
     if ((e = hipMalloc(&c->sc, sizeof(DevScalars))) != hipSuccess ||
         (e = hipHostMalloc(&c->sc_host, sizeof(DevScalars), hipHostMallocDefault)) != hipSuccess ||

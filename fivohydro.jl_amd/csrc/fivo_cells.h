@@ -176,13 +176,31 @@ __device__ __forceinline__ void prim_counts(DevScalars* sc, unsigned long long i
     }
 }
 
-// the warm part of the primitive pass for cell i; queued cells go to jobs[*njobs++]
-template <int EK, bool NCH>
+// list[(*cnt)++] = val for the lanes with `take`, one atomic per wavefront, the
+// lanes in order; all lanes of the wavefront call it
+__device__ __forceinline__ void wave_append(bool take, int val, unsigned long long* cnt, int* list) {
+    const unsigned long long m = __ballot(take);
+    if (m == 0ull) return;
+    const int lead = __builtin_ctzll(m);
+    unsigned long long base = 0;
+    if ((int)(threadIdx.x & 63) == lead) base = atomicAdd(cnt, (unsigned long long)__popcll(m));
+    base = __shfl(base, lead, 64);
+    if (take) list[base + __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u))] = val;
+}
+
+// the warm part of the primitive pass for cell i; queued cells go to jobs[*njobs++].
+// CHECK: only the cells whose warm start is already converged (or that end at their
+// first residual: vacuum, non-finite) are finished here; a cell that would iterate
+// sets *iterate and is left to a second launch over the compacted list of such
+// cells, which repeats this function without CHECK from the same inputs. About
+// 80 % of a warm pass's cells need no iteration and the rest 2-5, so wavefronts
+// of all cells would idle most of their lanes through every iteration.
+template <int EK, bool NCH, bool CHECK = false>
 __device__ __forceinline__ void prim_cell(const double* __restrict__ U, const Params& p, double tau,
                                           const DevWork& w, int i, int ev, unsigned long long* njobs,
                                           PrimJob* jobs, unsigned long long& its, unsigned long long& nfail,
                                           unsigned long long& nvac, unsigned long long& nq,
-                                          unsigned long long& nre) {
+                                          unsigned long long& nre, bool* iterate = nullptr) {
     const double D = safe_div(U[i], tau), Sx = U[P_ + i], Sy = U[2 * P_ + i], E = U[3 * P_ + i];
     const CellIn c = make_cell<NCH>(D, Sx, Sy, E, load_diss(U, p, i));
     const double yT0 = w.x0_yT[i], ux0 = w.x0_ux[i], uy0 = w.x0_uy[i];
@@ -195,7 +213,13 @@ __device__ __forceinline__ void prim_cell(const double* __restrict__ U, const Pa
         bool fin = newton_begin<EK, NCH, false, 0>(c, yT0, w.x0_phi[i], ux0, uy0, p.eos, s, r);
         bool paused = false;
         if (!fin) {
-            newton_iterate<EK, NCH, 0>(s, c, p.eos, PRIM_WARM_ITCAP);
+            if constexpr (CHECK) {
+                // newton_iterate's test at iteration 1: converged with no iteration
+                if (!(newton_scaled_norm<NCH>(s, c) < NEWTON_TOL)) { *iterate = true; return; }
+                s.converged = true; s.reason = PRR_CONVERGED; s.iters = s.it - 1;
+            } else {
+                newton_iterate<EK, NCH, 0>(s, c, p.eos, PRIM_WARM_ITCAP);
+            }
             paused = newton_paused(s);
             if (paused) {
                 job = s.it;
@@ -210,7 +234,7 @@ __device__ __forceinline__ void prim_cell(const double* __restrict__ U, const Pa
             }
         }
         if (!paused) {
-            its = r.iters;
+            its += r.iters;
             if (r.ok || r.reason == PRR_VACUUM) { prim_finalize(p, w, i, r, c.d, nfail, nvac); job = -1; }
             else if (warm) nre += 1;             // the warm start failed: the cell retries from the cold seed
         }

@@ -57,6 +57,7 @@ struct DevScalars {
     FIVO_SCALAR(shear_max_bits);
     FIVO_SCALAR(npending);      // near-bound cells awaiting the inversion test
     FIVO_SCALAR(ncold);         // cells of the current primitive pass queued for the cooperative solve
+    FIVO_SCALAR(nwarm);         // cells of the current primitive pass whose warm start iterates
     FIVO_SCALAR(nlearned);      // cumulative: cells started from the learned seed
     FIVO_SCALAR(nfallback);     // cumulative: of those, cells that fell back to the cold seed
     FIVO_SCALAR(nqueued);       // cumulative: cells queued for the cooperative kernel
@@ -102,6 +103,7 @@ struct Ctx {
     double* snap = nullptr;         // pre-relaxation snapshot (dissipation2d.jl:299-303)
     double* partials = nullptr;     // reduction partials
     PrimJob* jobs = nullptr;        // cells of a primitive pass queued for the cooperative solve
+    int* wlist = nullptr;           // cells of a primitive pass whose warm Newton iterates
     uint8_t* dirty = nullptr;       // face tiles touching a queued cell (one byte per tile of the smallest tile)
     int ntiles_max = 0;
     hipStream_t stream2 = nullptr;  // the face tiles that run beside the cooperative kernel

@@ -14,12 +14,27 @@ __global__ __launch_bounds__(256) void k_bc(double* __restrict__ U, Params p, in
     if (t < bc_nghost(p)) bc_cell(U, p, periodic, t);
 }
 
+// warm pass, first launch: every cell; the cells whose Newton iterates are listed
 template <int EK, bool NCH>
-__global__ __launch_bounds__(256) void k_prim(const double* __restrict__ U, Params p, double tau, DevWork w,
-                                              DevScalars* sc, PrimJob* jobs) {
+__global__ __launch_bounds__(256) void k_prim_check(const double* __restrict__ U, Params p, double tau, DevWork w,
+                                                    DevScalars* sc, PrimJob* jobs, int* wlist) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long its = 0, nfail = 0, nvac = 0, nq = 0, nre = 0;
-    if (i < p.Ntot) prim_cell<EK, NCH>(U, p, tau, w, i, 0, &sc->ncold, jobs, its, nfail, nvac, nq, nre);
+    bool iterate = false;
+    if (i < p.Ntot)
+        prim_cell<EK, NCH, true>(U, p, tau, w, i, 0, &sc->ncold, jobs, its, nfail, nvac, nq, nre, &iterate);
+    wave_append(iterate, i, &sc->nwarm, wlist);
+    prim_counts(sc, its, nfail, nvac, nq);
+    rare_counts(sc, nre, 0);
+}
+// warm pass, second launch: the listed cells, densely packed into wavefronts
+template <int EK, bool NCH>
+__global__ __launch_bounds__(256) void k_prim(const double* __restrict__ U, Params p, double tau, DevWork w,
+                                              DevScalars* sc, PrimJob* jobs, const int* __restrict__ wlist) {
+    const int n = (int)sc->nwarm;
+    unsigned long long its = 0, nfail = 0, nvac = 0, nq = 0, nre = 0;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
+        prim_cell<EK, NCH>(U, p, tau, w, wlist[k], 0, &sc->ncold, jobs, its, nfail, nvac, nq, nre);
     prim_counts(sc, its, nfail, nvac, nq);
     rare_counts(sc, nre, 0);
 }
@@ -240,9 +255,12 @@ hipError_t launch_bc(Ctx& c, double* U, int bc) {
 
 hipError_t launch_prim_warm(Ctx& c, const double* U, double tau) {
     const Params& p = c.p;
-    // the worklist counter is cleared with the other per-pass scalars
-    EOS_DISPATCH(c, hipLaunchKernelGGL((k_prim<EK, NCH>), dim3(nblk(p.Ntot, 256)), dim3(256), 0, c.stream, U, p,
-                                       tau, c.w, c.sc, c.jobs));
+    // the queue and list counters are cleared with the other per-pass scalars; the
+    // iterating cells (~20 % of a warm pass) on a grid that strides over any count
+    EOS_DISPATCH(c, hipLaunchKernelGGL((k_prim_check<EK, NCH>), dim3(nblk(p.Ntot, 256)), dim3(256), 0, c.stream,
+                                       U, p, tau, c.w, c.sc, c.jobs, c.wlist));
+    EOS_DISPATCH(c, hipLaunchKernelGGL((k_prim<EK, NCH>), dim3(std::min(nblk(p.Ntot, 256), 1024)), dim3(256), 0,
+                                       c.stream, U, p, tau, c.w, c.sc, c.jobs, c.wlist));
     return hipGetLastError();
 }
 hipError_t launch_prim_seed(Ctx& c, const double* U, double tau) {
