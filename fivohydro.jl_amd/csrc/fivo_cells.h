@@ -491,14 +491,13 @@ __device__ __forceinline__ void mark_tiles(int Nytot, int ng, int i, uint8_t* di
         for (int b = blo; b <= bhi; ++b) dirty[a * tiles_y + b] = 1;
 }
 
-template <int TX, int TY>
+template <int TX, int TY, int NT = face_threads<TX, TY>()>
 __host__ __device__ inline void face_grid(const Params& p, int* tiles_y, int* nmain, int* nblocks) {
     const int tiles_x = (p.Nx + TX - 1) / TX;
     *tiles_y = (p.Ny + TY - 1) / TY;
     *nmain = tiles_x * *tiles_y;
     const int ng = p.ng;
     const int nextra = 2 * ng * (p.Nxtot - 1) + 2 * p.Ny * (ng - 1) + 2 * ng * (p.Nytot - 1) + 2 * p.Nx * (ng - 1);
-    constexpr int NT = face_threads<TX, TY>();
     *nblocks = *nmain + (nextra + NT - 1) / NT;
 }
 
@@ -712,9 +711,11 @@ __device__ __forceinline__ void store_cell(double* __restrict__ U, const Params&
 
 // ---- the fused face / update kernel ---------------------------------------------------
 // MODE 0: write the RHS to f.dUout (rhs_2d!); MODE 1: the RK stage update
-template <int EK, bool NCH, int TX, int TY, int MODE>
+// NT threads per block: face_threads (two faces per thread), or twice that where
+// only a few tiles run (the second pass beside the cooperative solve), so the face
+// phase takes one round and the block's latency shrinks
+template <int EK, bool NCH, int TX, int TY, int MODE, int NT = face_threads<TX, TY>()>
 __device__ __forceinline__ void faces_body(const FaceK& K, const int bid, double* __restrict__ sF) {
-    constexpr int NT = face_threads<TX, TY>();
     constexpr int NFX = (TX + 1) * TY, NFY = TX * (TY + 1), NF = NFX + NFY;
     const Params& p = K.p;
     const int tid = threadIdx.x;

@@ -73,6 +73,12 @@ __global__ __launch_bounds__((face_threads<TX, TY>())) void k_faces(FaceK K) {
     extern __shared__ double sF[];
     faces_body<EK, NCH, TX, TY, MODE>(K, blockIdx.x, sF);
 }
+// the same with one face per thread (the few tiles of the second pass)
+template <int EK, bool NCH, int TX, int TY, int MODE>
+__global__ __launch_bounds__((2 * face_threads<TX, TY>())) void k_faces_wide(FaceK K) {
+    extern __shared__ double sF[];
+    faces_body<EK, NCH, TX, TY, MODE, 2 * face_threads<TX, TY>()>(K, blockIdx.x, sF);
+}
 
 template <int EK, bool NCH>
 __global__ __launch_bounds__(256) void k_admissible_pending(const double* U, Params p, double tau, uint8_t* bad,
@@ -304,7 +310,12 @@ static hipError_t faces_tx(Ctx& c, const FaceLaunch& f, int pass, hipStream_t st
     FaceK K;
     K.p = p; K.f = f; K.w = c.w; K.sc = c.sc; K.remap = 1; K.pass = pass; K.dirty = c.dirty;
     int nb;
-    face_grid<TX, TY>(p, &K.tiles_y, &K.nmain, &nb);
+    constexpr int NTW = 2 * face_threads<TX, TY>();
+    // the second pass (the ~10 % of tiles near a queued cell, after the cooperative
+    // solve) runs one face per thread: its blocks are few, so their latency counts
+    const bool wide = pass == 2;
+    if (wide) face_grid<TX, TY, NTW>(p, &K.tiles_y, &K.nmain, &nb);
+    else face_grid<TX, TY>(p, &K.tiles_y, &K.nmain, &nb);
     constexpr int NF = (TX + 1) * TY + TX * (TY + 1);
     const int nvf = p.iPQ0 >= 0 ? p.iPQ0 : p.nv;   // rows with a face flux
     const size_t lds = sizeof(double) * (size_t)nvf * NF;
@@ -315,11 +326,22 @@ static hipError_t faces_tx(Ctx& c, const FaceLaunch& f, int pass, hipStream_t st
         if (e == hipSuccess)
             e = hipFuncSetAttribute((const void*)k_faces<EK, NCH, TX, TY, 1>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute((const void*)k_faces_wide<EK, NCH, TX, TY, 0>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute((const void*)k_faces_wide<EK, NCH, TX, TY, 1>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
         lds_set = lds;
     }
-    if (f.mode == 0) hipLaunchKernelGGL((k_faces<EK, NCH, TX, TY, 0>), dim3(nb), dim3(face_threads<TX, TY>()), lds, st, K);
-    else hipLaunchKernelGGL((k_faces<EK, NCH, TX, TY, 1>), dim3(nb), dim3(face_threads<TX, TY>()), lds, st, K);
+    if (wide) {
+        if (f.mode == 0) hipLaunchKernelGGL((k_faces_wide<EK, NCH, TX, TY, 0>), dim3(nb), dim3(NTW), lds, st, K);
+        else hipLaunchKernelGGL((k_faces_wide<EK, NCH, TX, TY, 1>), dim3(nb), dim3(NTW), lds, st, K);
+    } else {
+        if (f.mode == 0) hipLaunchKernelGGL((k_faces<EK, NCH, TX, TY, 0>), dim3(nb), dim3(face_threads<TX, TY>()), lds, st, K);
+        else hipLaunchKernelGGL((k_faces<EK, NCH, TX, TY, 1>), dim3(nb), dim3(face_threads<TX, TY>()), lds, st, K);
+    }
     return hipGetLastError();
 }
 
