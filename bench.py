@@ -653,6 +653,23 @@ def cpu_baseline(cfg, budget_s=12.0, background=False):
     return out
 
 
+def primitive_histograms(lib, s, r, cfg):
+    """Where a primitive pass's Newton work goes (fivo_get_histograms over the timed
+    run), per pass: every cell of the warm pass by warm iterations (0 = finished at
+    its first residual; queued = left to the cooperative solve), and the cooperative
+    solves by total iterations, resumed warm solves apart from seeded ones."""
+    h = (C.c_int64 * 24)()
+    lib.get_histograms(s.ctx, h)
+    h = [int(x) for x in h]
+    passes = r["nrhs"] + (r["nsteps"] if cfg.get("relax") else 0)
+    bins = [f"{10 * k}-{10 * k + 9}" for k in range(7)] + ["70+"]
+    return {"passes": passes,
+            "warm_cells_per_pass": dict(zip(["first residual", "1", "2", "3", "4", "5", "6", "queued"],
+                                            [round(x / passes, 1) for x in h[0:8]])),
+            "coop_resumed_per_pass_by_iterations": dict(zip(bins, [round(x / passes, 1) for x in h[8:16]])),
+            "coop_seeded_per_pass_by_iterations": dict(zip(bins, [round(x / passes, 1) for x in h[16:24]]))}
+
+
 def kernel_times(lib, ctx):
     """{kernel: (total ms, launches)} from the library's HIP-event timers (one
     event pair around each launch on the library stream)."""
@@ -698,6 +715,7 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     r_plain = r
+    hist = primitive_histograms(lib, s, r, cfg)
     Tp = np.exp(s.get_work(("yT",))["yT"])
     value = a.steps * ncell / wall
     log(f"{cfg['workload']} plain: {value:.3e} cell-updates/s")
@@ -729,6 +747,7 @@ def main():
                    "plain_ms_per_step_kernel_timed": wall_k / a.steps * 1e3,
                    "alg_bytes_per_cell_update": algorithmic_bytes(cfg),
                    "kernel_ms": {k: round(v[0] / v[1], 5) for k, v in tms.items()},
+                   "primitive_pass_histogram": hist,
                    "kernel_launches": {k: v[1] for k, v in tms.items()}},
         "roofline": {"bound": "hbm", "kernel": dom, "path": "plain", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC.get((a.config, dom)),

@@ -1067,6 +1067,13 @@ int fivo_probe_m2_source(uint32_t terms, int64_t n, const double* in, double* ou
     return probe(1, terms, n, in, out, "fivo_probe_m2_source");
 }
 
+int fivo_get_histograms(const fivo_ctx* ctx, int64_t* counts) {
+    const Ctx& c = *reinterpret_cast<const Ctx*>(ctx);
+    if (!counts) return FIVO_ERR_ARG;
+    for (int k = 0; k < 24; ++k) counts[k] = (int64_t)c.sc_host->hist[k][0];
+    return FIVO_OK;
+}
+
 int64_t fivo_kernel_time(const fivo_ctx* ctx, int cat, double* total_ms) {
     const Ctx& c = *reinterpret_cast<const Ctx*>(ctx);
     if (cat < 0 || cat >= TK_N) return -1;

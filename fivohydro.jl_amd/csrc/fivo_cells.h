@@ -188,6 +188,19 @@ __device__ __forceinline__ void wave_append(bool take, int val, unsigned long lo
     if (take) list[base + __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u))] = val;
 }
 
+// Histogram bins of a block's threads (bin < 0: none), summed in LDS and added to
+// sc->hist[base + bin] with one atomic per nonzero bin. Every thread of the block
+// calls it.
+__device__ __forceinline__ void block_hist(DevScalars* sc, int base, int bin0, int bin1 = -1) {
+    __shared__ unsigned int h[16];
+    if (threadIdx.x < 16) h[threadIdx.x] = 0u;
+    __syncthreads();
+    if (bin0 >= 0) atomicAdd(&h[bin0], 1u);
+    if (bin1 >= 0) atomicAdd(&h[bin1], 1u);
+    __syncthreads();
+    if (threadIdx.x < 16 && h[threadIdx.x]) atomicAdd(&sc->hist[base + threadIdx.x][0], (unsigned long long)h[threadIdx.x]);
+}
+
 // the warm part of the primitive pass for cell i; queued cells go to jobs[*njobs++].
 // CHECK: only the cells whose warm start is already converged (or that end at their
 // first residual: vacuum, non-finite) are finished here; a cell that would iterate
@@ -200,13 +213,14 @@ __device__ __forceinline__ void prim_cell(const double* __restrict__ U, const Pa
                                           const DevWork& w, int i, int ev, unsigned long long* njobs,
                                           PrimJob* jobs, unsigned long long& its, unsigned long long& nfail,
                                           unsigned long long& nvac, unsigned long long& nq,
-                                          unsigned long long& nre, bool* iterate = nullptr) {
+                                          unsigned long long& nre, bool* iterate = nullptr, int* hbin = nullptr) {
     const double D = safe_div(U[i], tau), Sx = U[P_ + i], Sy = U[2 * P_ + i], E = U[3 * P_ + i];
     const CellIn c = make_cell<NCH>(D, Sx, Sy, E, load_diss(U, p, i));
     const double yT0 = w.x0_yT[i], ux0 = w.x0_ux[i], uy0 = w.x0_uy[i];
     const bool vac = !isf(E) || E <= E_VAC || !isf(D) || D < 0.0;
     const bool warm = isf(yT0) && isf(ux0) && isf(uy0);
     int job = 0;                                 // < 0: done here; 0: cold seed; > 0: resume
+    int cell_iters = 0;
     if (vac || warm) {
         NewtonState s;
         PrimRes r;
@@ -235,10 +249,12 @@ __device__ __forceinline__ void prim_cell(const double* __restrict__ U, const Pa
         }
         if (!paused) {
             its += r.iters;
+            cell_iters = r.iters;
             if (r.ok || r.reason == PRR_VACUUM) { prim_finalize(p, w, i, r, c.d, nfail, nvac); job = -1; }
             else if (warm) nre += 1;             // the warm start failed: the cell retries from the cold seed
         }
     }
+    if (hbin) *hbin = job >= 0 ? 7 : (CHECK ? 0 : min(cell_iters, 6));
     if (job >= 0) {
         const unsigned long long k = atomicAdd(njobs, 1ull);
         jobs[k] = PrimJob{i, job, ev};

@@ -65,6 +65,12 @@ struct DevScalars {
     FIVO_SCALAR(bg_vmax_bits);
     FIVO_SCALAR(nretry);        // cumulative: warm-started solves that failed and were retried from the cold seed
     FIVO_SCALAR(nstaged);       // cumulative: solves closed by the staged phi fallback (bisection of the charge row)
+    // cumulative histograms of the primitive passes (fivo_get_histograms), each bin on
+    // its own line: [0, 8) every cell of a warm pass -- 0 finished at its first
+    // residual, 1-6 converged after that many warm iterations, 7 queued for the
+    // cooperative solve; [8, 16) the cooperative solves that resumed a warm solve and
+    // [16, 24) those started from a seed, by total Newton iterations in bins of ten
+    unsigned long long hist[24][16];
 };
 #undef FIVO_SCALAR
 

@@ -21,9 +21,11 @@ __global__ __launch_bounds__(256) void k_prim_check(const double* __restrict__ U
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long its = 0, nfail = 0, nvac = 0, nq = 0, nre = 0;
     bool iterate = false;
+    int bin = -1;
     if (i < p.Ntot)
-        prim_cell<EK, NCH, true>(U, p, tau, w, i, 0, &sc->ncold, jobs, its, nfail, nvac, nq, nre, &iterate);
+        prim_cell<EK, NCH, true>(U, p, tau, w, i, 0, &sc->ncold, jobs, its, nfail, nvac, nq, nre, &iterate, &bin);
     wave_append(iterate, i, &sc->nwarm, wlist);
+    block_hist(sc, 0, iterate ? -1 : bin);
     prim_counts(sc, its, nfail, nvac, nq);
     rare_counts(sc, nre, 0);
 }
@@ -33,8 +35,13 @@ __global__ __launch_bounds__(256) void k_prim(const double* __restrict__ U, Para
                                               DevScalars* sc, PrimJob* jobs, const int* __restrict__ wlist) {
     const int n = (int)sc->nwarm;
     unsigned long long its = 0, nfail = 0, nvac = 0, nq = 0, nre = 0;
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
-        prim_cell<EK, NCH>(U, p, tau, w, wlist[k], 0, &sc->ncold, jobs, its, nfail, nvac, nq, nre);
+    int bin0 = -1, bin1 = -1;   // the grid holds the list of a warm pass; a second cell per thread is rare
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        int bin = -1;
+        prim_cell<EK, NCH>(U, p, tau, w, wlist[k], 0, &sc->ncold, jobs, its, nfail, nvac, nq, nre, nullptr, &bin);
+        if (bin0 < 0) bin0 = bin; else if (bin1 < 0) bin1 = bin; else atomicAdd(&sc->hist[bin][0], 1ull);
+    }
+    block_hist(sc, 0, bin0, bin1);
     prim_counts(sc, its, nfail, nvac, nq);
     rare_counts(sc, nre, 0);
 }
@@ -45,11 +52,18 @@ __global__ __launch_bounds__(256) void k_prim_coop(const double* __restrict__ U,
     const int n = (int)sc->ncold;
     const int ngroups = gridDim.x * (blockDim.x / G);
     unsigned long long its = 0, nfail = 0, nvac = 0, nfb = 0, nre = 0, nst = 0;
+    int bin0 = -1, bin1 = -1;
     for (int t = (blockIdx.x * blockDim.x + threadIdx.x) / G; t < n; t += ngroups) {
         int itsum;
-        coop_job<EK, NCH, G>(U, p, tau, w, jobs[t], itsum, nfail, nvac, nfb, nre, nst);
-        if (glane<G>() == 0) its += itsum;
+        const PrimJob jb = jobs[t];
+        coop_job<EK, NCH, G>(U, p, tau, w, jb, itsum, nfail, nvac, nfb, nre, nst);
+        if (glane<G>() == 0) {
+            its += itsum;
+            const int bin = (jb.it > 0 ? 0 : 8) + min(itsum / 10, 7);   // resumed / seeded solves
+            if (bin0 < 0) bin0 = bin; else if (bin1 < 0) bin1 = bin; else atomicAdd(&sc->hist[8 + bin][0], 1ull);
+        }
     }
+    block_hist(sc, 8, bin0, bin1);
     prim_counts(sc, its, nfail, nvac);
     if (glane<G>() == 0 && nfb) atomicAdd(&sc->nfallback, nfb);
     const bool lead = glane<G>() == 0;

@@ -191,6 +191,7 @@ def load_hip():
         lib = Lib(HIP_LIB_PATH, "fivo_")
         for name, res, args in (("kernel_time", C.c_int64, [C.c_void_p, C.c_int, _DP]),
                                 ("enable_timing", C.c_int, [C.c_void_p, C.c_int]),
+                                ("get_histograms", C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
                                 ("batch_create", C.c_int, [C.POINTER(GridDesc), C.c_int32, C.POINTER(ModelDesc),
                                                            C.POINTER(C.c_void_p)]),
                                 ("batch_destroy", None, [C.c_void_p]),

@@ -251,6 +251,14 @@ int fivo_background_slice(fivo_ctx* ctx, int32_t k, int32_t nt, double* T, doubl
  * of launches. */
 int64_t fivo_kernel_time(const fivo_ctx* ctx, int category, double* total_ms);
 int fivo_enable_timing(fivo_ctx* ctx, int on);
+/* Histograms of the primitive passes of the last fivo_run (cumulative over the run),
+ * 24 counts: [0, 8) every cell of every warm pass -- 0 finished at its first
+ * residual (converged warm start, vacuum), 1-6 converged after that many warm
+ * Newton iterations, 7 queued for the cooperative solve; [8, 16) the cooperative
+ * solves that resumed a warm solve and [16, 24) those started from a seed (cold or
+ * learned), by total Newton iterations in bins of ten (the last bin: 70 or more).
+ * No counterpart in the reference: a diagnostic of where the pass's time goes. */
+int fivo_get_histograms(const fivo_ctx* ctx, int64_t* counts);
 
 /* ---- probes of the relaxation's source terms ---------------------------------- */
 /* Evaluate the device code of a source term on n states given row by row (host
