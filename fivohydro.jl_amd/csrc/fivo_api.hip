@@ -110,8 +110,7 @@ hipError_t clear_scalars(Ctx& c, unsigned long long* first, unsigned long long* 
 }
 int reset_scalars(Ctx& c) {
     // amax, nbad, nprimfail, nvacuum, npending, ncold, nwarm -> 0 (newton_iters kept)
-    HIPCHK(&c, clear_scalars(c, &c.sc->amax_bits, &c.sc->nvacuum));
-    HIPCHK(&c, clear_scalars(c, &c.sc->npending, &c.sc->nwarm));
+    HIPCHK(&c, clear_scalars(c, &c.sc->amax_bits, &c.sc->nwarm));
     return FIVO_OK;
 }
 int read_scalars(Ctx& c) {

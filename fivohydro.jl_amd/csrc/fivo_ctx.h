@@ -49,15 +49,17 @@ struct Params {
 // MI355X), so counters sharing a line would also wait for each other's updates.
 #define FIVO_SCALAR(name) unsigned long long name; unsigned long long name##_line_[15]
 struct DevScalars {
+    // cleared before each primitive pass (reset_scalars): amax_bits .. nwarm
     FIVO_SCALAR(amax_bits);     // max |signal speed| of the last RHS (positive double bits)
     FIVO_SCALAR(nbad);          // inadmissible cells of the last stage
     FIVO_SCALAR(nprimfail);     // last primitive pass
     FIVO_SCALAR(nvacuum);
-    FIVO_SCALAR(newton_iters);  // cumulative
-    FIVO_SCALAR(shear_max_bits);
     FIVO_SCALAR(npending);      // near-bound cells awaiting the inversion test
     FIVO_SCALAR(ncold);         // cells of the current primitive pass queued for the cooperative solve
     FIVO_SCALAR(nwarm);         // cells of the current primitive pass whose warm start iterates
+    // cumulative
+    FIVO_SCALAR(newton_iters);
+    FIVO_SCALAR(shear_max_bits);
     FIVO_SCALAR(nlearned);      // cumulative: cells started from the learned seed
     FIVO_SCALAR(nfallback);     // cumulative: of those, cells that fell back to the cold seed
     FIVO_SCALAR(nqueued);       // cumulative: cells queued for the cooperative kernel
