@@ -109,7 +109,7 @@ hipError_t clear_scalars(Ctx& c, unsigned long long* first, unsigned long long* 
     return hipMemsetAsync(first, 0, (size_t)((char*)(last + 1) - (char*)first), c.stream);
 }
 int reset_scalars(Ctx& c) {
-    // amax, nbad, nprimfail, nvacuum, npending, ncold, nwarm -> 0 (newton_iters kept)
+    // amax, nbad, nprimfail, nvacuum, npending, ncold, nresq, nwarm -> 0 (newton_iters kept)
     HIPCHK(&c, clear_scalars(c, &c.sc->amax_bits, &c.sc->nwarm));
     return FIVO_OK;
 }

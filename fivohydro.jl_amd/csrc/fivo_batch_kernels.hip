@@ -246,7 +246,7 @@ __global__ __launch_bounds__(256) void kb_project_shear(BatchRun R, int correct)
 // what: 2 run start, 0 after the initial RHS, 1 step begin, 10+s after stage s,
 // 20 after relaxation, 21 step end
 __device__ __forceinline__ void reset_pass(DevScalars* sc) {
-    sc->amax_bits = 0; sc->nbad = 0; sc->nprimfail = 0; sc->nvacuum = 0; sc->npending = 0; sc->ncold = 0; sc->nwarm = 0;
+    sc->amax_bits = 0; sc->nbad = 0; sc->nprimfail = 0; sc->nvacuum = 0; sc->npending = 0; sc->ncold = 0; sc->nresq = 0; sc->nwarm = 0;
 }
 __global__ __launch_bounds__(64) void kb_ctl(BatchRun R, int what) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;

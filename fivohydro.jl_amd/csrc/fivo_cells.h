@@ -213,7 +213,8 @@ __device__ __forceinline__ void prim_cell(const double* __restrict__ U, const Pa
                                           const DevWork& w, int i, int ev, unsigned long long* njobs,
                                           PrimJob* jobs, unsigned long long& its, unsigned long long& nfail,
                                           unsigned long long& nvac, unsigned long long& nq,
-                                          unsigned long long& nre, bool* iterate = nullptr, int* hbin = nullptr) {
+                                          unsigned long long& nre, bool* iterate = nullptr, int* hbin = nullptr,
+                                          unsigned long long* nres = nullptr, int qcap = 0) {
     const double D = safe_div(U[i], tau), Sx = U[P_ + i], Sy = U[2 * P_ + i], E = U[3 * P_ + i];
     const CellIn c = make_cell<NCH>(D, Sx, Sy, E, load_diss(U, p, i));
     const double yT0 = w.x0_yT[i], ux0 = w.x0_ux[i], uy0 = w.x0_uy[i];
@@ -256,8 +257,17 @@ __device__ __forceinline__ void prim_cell(const double* __restrict__ U, const Pa
     }
     if (hbin) *hbin = job >= 0 ? 7 : (CHECK ? 0 : min(cell_iters, 6));
     if (job >= 0) {
-        const unsigned long long k = atomicAdd(njobs, 1ull);
-        jobs[k] = PrimJob{i, job, ev};
+        // with nres: resumed solves fill the queue from its far end (jobs[qcap - 1]
+        // down), seeded ones from the front, so the cooperative kernel pairs like
+        // with like in a wavefront and a ~56-iteration seeded walk never shares one
+        // with a resumed solve's divergent branches
+        if (nres && job > 0) {
+            const unsigned long long k = atomicAdd(nres, 1ull);
+            jobs[qcap - 1 - (long long)k] = PrimJob{i, job, ev};
+        } else {
+            const unsigned long long k = atomicAdd(njobs, 1ull);
+            jobs[k] = PrimJob{i, job, ev};
+        }
         nq += 1;
     }
 }

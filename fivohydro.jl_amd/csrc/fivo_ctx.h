@@ -56,6 +56,7 @@ struct DevScalars {
     FIVO_SCALAR(nvacuum);
     FIVO_SCALAR(npending);      // near-bound cells awaiting the inversion test
     FIVO_SCALAR(ncold);         // cells of the current primitive pass queued for the cooperative solve
+    FIVO_SCALAR(nresq);         // of a plain pass's queue, resumed solves (at the queue's far end)
     FIVO_SCALAR(nwarm);         // cells of the current primitive pass whose warm start iterates
     // cumulative
     FIVO_SCALAR(newton_iters);

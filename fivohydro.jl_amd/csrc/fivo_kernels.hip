@@ -23,7 +23,8 @@ __global__ __launch_bounds__(256) void k_prim_check(const double* __restrict__ U
     bool iterate = false;
     int bin = -1;
     if (i < p.Ntot)
-        prim_cell<EK, NCH, true>(U, p, tau, w, i, 0, &sc->ncold, jobs, its, nfail, nvac, nq, nre, &iterate, &bin);
+        prim_cell<EK, NCH, true>(U, p, tau, w, i, 0, &sc->ncold, jobs, its, nfail, nvac, nq, nre, &iterate, &bin,
+                                 &sc->nresq, p.Ntot);
     wave_append(iterate, i, &sc->nwarm, wlist);
     block_hist(sc, 0, iterate ? -1 : bin);
     prim_counts(sc, its, nfail, nvac, nq);
@@ -38,7 +39,8 @@ __global__ __launch_bounds__(256) void k_prim(const double* __restrict__ U, Para
     int bin0 = -1, bin1 = -1;   // the grid holds the list of a warm pass; a second cell per thread is rare
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
         int bin = -1;
-        prim_cell<EK, NCH>(U, p, tau, w, wlist[k], 0, &sc->ncold, jobs, its, nfail, nvac, nq, nre, nullptr, &bin);
+        prim_cell<EK, NCH>(U, p, tau, w, wlist[k], 0, &sc->ncold, jobs, its, nfail, nvac, nq, nre, nullptr, &bin,
+                           &sc->nresq, p.Ntot);
         if (bin0 < 0) bin0 = bin; else if (bin1 < 0) bin1 = bin; else atomicAdd(&sc->hist[bin][0], 1ull);
     }
     block_hist(sc, 0, bin0, bin1);
@@ -49,13 +51,14 @@ __global__ __launch_bounds__(256) void k_prim(const double* __restrict__ U, Para
 template <int EK, bool NCH, int G>
 __global__ __launch_bounds__(256) void k_prim_coop(const double* __restrict__ U, Params p, double tau, DevWork w,
                                                    DevScalars* sc, const PrimJob* __restrict__ jobs) {
-    const int n = (int)sc->ncold;
+    // seeded solves at the queue's front, resumed ones at its far end (prim_cell)
+    const int nf = (int)sc->ncold, n = nf + (int)sc->nresq;
     const int ngroups = gridDim.x * (blockDim.x / G);
     unsigned long long its = 0, nfail = 0, nvac = 0, nfb = 0, nre = 0, nst = 0;
     int bin0 = -1, bin1 = -1;
     for (int t = (blockIdx.x * blockDim.x + threadIdx.x) / G; t < n; t += ngroups) {
         int itsum;
-        const PrimJob jb = jobs[t];
+        const PrimJob jb = t < nf ? jobs[t] : jobs[p.Ntot - 1 - (t - nf)];
         coop_job<EK, NCH, G>(U, p, tau, w, jb, itsum, nfail, nvac, nfb, nre, nst);
         if (glane<G>() == 0) {
             its += itsum;
@@ -373,9 +376,9 @@ template <int TX, int TY>
 __global__ __launch_bounds__(256) void k_mark_tiles(Params p, const DevScalars* __restrict__ sc,
                                                     const PrimJob* __restrict__ jobs, uint8_t* dirty, int tiles_y,
                                                     int nmain) {
-    const int n = (int)sc->ncold;
+    const int nf = (int)sc->ncold, n = nf + (int)sc->nresq;
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x)
-        mark_tiles<TX, TY>(p.Nytot, p.ng, jobs[t].cell, dirty, tiles_y, nmain);
+        mark_tiles<TX, TY>(p.Nytot, p.ng, (t < nf ? jobs[t] : jobs[p.Ntot - 1 - (t - nf)]).cell, dirty, tiles_y, nmain);
 }
 template <int TX, int TY>
 static hipError_t mark_tx(Ctx& c) {
