@@ -57,16 +57,17 @@ def log(msg):
 # them); keyed by (config, kernel). The figures are those of the serialised
 # launches the HIP events time (cooperative kernel [alone], face kernel [one pass]).
 TRAFFIC = {
-    ("c3", "k_prim_coop"): (2.0 * 1131.6 + 1509.9) * 1024,
-    ("c3", "k_prim"): (2.0 * 15274.2 + 33818.9) * 1024,
-    ("c3", "k_faces"): (2.0 * 42034.6 + 34280.4) * 1024,
-    ("c3", "k_relax"): (2.0 * 45013.4 + 67113.7) * 1024,
+    ("c3", "k_prim_coop"): (2.0 * 1287.1 + 1576.2) * 1024,
+    # the warm pass: k_prim_check over every cell, then k_prim over the iterating ones
+    ("c3", "k_prim"): (2.0 * (15105.2 + 4019.3) + (27526.8 + 6943.6)) * 1024,
+    ("c3", "k_faces"): (2.0 * 42366.4 + 34257.4) * 1024,
+    ("c3", "k_relax"): (2.0 * 44982.5 + 67038.4) * 1024,
     ("c3", "k_bc"): (2.0 * 1254.5 + 1159.0) * 1024,
 }
 
 TRAFFIC_SOURCE = ("profile constant: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this workload, "
                   "scaled by a 512 MiB device copy (profiles/full_sectors_512_kernel_stats.md)")
-FP64_GFLOP = {("c3", "k_prim_coop"): 0.865, ("c3", "k_prim"): 0.909, ("c3", "k_faces"): 1.347,
+FP64_GFLOP = {("c3", "k_prim_coop"): 0.718, ("c3", "k_prim"): 0.150 + 0.609, ("c3", "k_faces"): 1.347,
               ("c3", "k_relax"): 0.554, ("c3", "k_bc"): 0.0}
 FP64_PEAK_TFLOPS = 78.6
 
