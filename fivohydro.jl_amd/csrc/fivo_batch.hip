@@ -191,7 +191,7 @@ int fivo_batch_create(const fivo_grid_desc* grids, int32_t nev, const fivo_model
         Ctx& c = *b->evs[k];
         EvView& v = b->h_ev[k];
         std::memset(&v, 0, sizeof(EvView));
-        v.p = c.p; v.w = c.w; v.sc = c.sc; v.partials = c.partials; v.jobs = c.jobs;
+        v.p = c.p; v.w = c.w; v.sc = c.sc; v.partials = c.partials; v.jobs = c.jobs; v.wlist = c.wlist;
         for (int r = 0; r < 4; ++r) v.U[r] = c.U[r];
         cells += (int64_t)c.p.Nx * c.p.Ny;
     }

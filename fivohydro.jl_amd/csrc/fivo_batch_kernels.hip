@@ -74,8 +74,10 @@ __global__ __launch_bounds__(256) void kb_bc(BatchRun R, int phase, int role) {
 }
 
 // ---- primitive pass: warm part, one lane per cell --------------------------------------
+// first launch: every cell's warm start is tested; converged and vacuum cells are
+// finished, the cells whose Newton iterates are listed (per event)
 template <int EK, bool NCH>
-__global__ __launch_bounds__(256) void kb_prim(BatchRun R, int phase) {
+__global__ __launch_bounds__(256) void kb_prim_check(BatchRun R, int phase) {
     const int e = block_event(R, KB_CELLS);
     CCtl c = (CCtl)(R.ctl + e);
     if (!ev_active(c, phase)) return;
@@ -84,11 +86,40 @@ __global__ __launch_bounds__(256) void kb_prim(BatchRun R, int phase) {
     const DevWork w = ev_work(E);
     const double* U = E->U[(c->cur + phase_in_role(phase)) & 3];
     const double tau = phase_tau(c, phase);
+    DevScalars* sc = E->sc;
     const int i = (blockIdx.x - E->b0[KB_CELLS]) * 256 + threadIdx.x;
     unsigned long long its = 0, nfail = 0, nvac = 0, nq = 0, nre = 0;
-    if (i < p.Ntot) prim_cell<EK, NCH>(U, p, tau, w, i, e, &E->sc->ncold, E->jobs, its, nfail, nvac, nq, nre);
-    prim_counts(E->sc, its, nfail, nvac, nq);
-    rare_counts(E->sc, nre, 0);
+    bool iterate = false;
+    if (i < p.Ntot)
+        prim_cell<EK, NCH, true>(U, p, tau, w, i, e, &sc->ncold, E->jobs, its, nfail, nvac, nq, nre, &iterate,
+                                 nullptr, &sc->nresq, p.Ntot);
+    wave_append(iterate, i, &sc->nwarm, E->wlist);
+    prim_counts(sc, its, nfail, nvac, nq);
+    rare_counts(sc, nre, 0);
+}
+// second launch: the event's listed cells, densely packed into wavefronts (the
+// list is at most the event's cell count, so one cell per thread; blocks past it exit)
+template <int EK, bool NCH>
+__global__ __launch_bounds__(256) void kb_prim(BatchRun R, int phase) {
+    const int e = block_event(R, KB_CELLS);
+    CCtl c = (CCtl)(R.ctl + e);
+    if (!ev_active(c, phase)) return;
+    CEv E = (CEv)(R.ev + e);
+    DevScalars* sc = E->sc;
+    const int n = (int)sc->nwarm;
+    const int k0 = (blockIdx.x - E->b0[KB_CELLS]) * 256;
+    if (k0 >= n) return;
+    const Params p = ev_params(R, E);
+    const DevWork w = ev_work(E);
+    const double* U = E->U[(c->cur + phase_in_role(phase)) & 3];
+    const double tau = phase_tau(c, phase);
+    const int k = k0 + threadIdx.x;
+    unsigned long long its = 0, nfail = 0, nvac = 0, nq = 0, nre = 0;
+    if (k < n)
+        prim_cell<EK, NCH>(U, p, tau, w, E->wlist[k], e, &sc->ncold, E->jobs, its, nfail, nvac, nq, nre, nullptr,
+                           nullptr, &sc->nresq, p.Ntot);
+    prim_counts(sc, its, nfail, nvac, nq);
+    rare_counts(sc, nre, 0);
 }
 
 // ---- primitive pass: the event's queued cells, G lanes per cell -----------------------------
@@ -103,7 +134,8 @@ __global__ __launch_bounds__(64, G == 8 ? 2 : 1) void kb_coop(BatchRun R, int ph
     CCtl c = (CCtl)(R.ctl + e);
     if (!ev_active(c, phase)) return;
     CEv E = (CEv)(R.ev + e);
-    const int n = (int)E->sc->ncold;
+    // seeded solves at the queue's front, resumed ones at its far end (prim_cell)
+    const int nf = (int)E->sc->ncold, n = nf + (int)E->sc->nresq;
     const int lb = blockIdx.x - E->b0[KB_COOP];
     const int ngroups = E->nb[KB_COOP] * (64 / G);
     int t = (lb * 64 + (int)threadIdx.x) / G;
@@ -116,7 +148,8 @@ __global__ __launch_bounds__(64, G == 8 ? 2 : 1) void kb_coop(BatchRun R, int ph
     unsigned long long its = 0, nfail = 0, nvac = 0, nfb = 0, nre = 0, nst = 0;
     for (; t < n; t += ngroups) {
         int itsum;
-        coop_job<EK, NCH, G>(U, p, tau, w, jobs[t], itsum, nfail, nvac, nfb, nre, nst);
+        coop_job<EK, NCH, G>(U, p, tau, w, t < nf ? jobs[t] : jobs[p.Ntot - 1 - (t - nf)], itsum, nfail, nvac, nfb,
+                             nre, nst);
         if (glane<G>() == 0) its += itsum;
     }
     prim_counts(E->sc, its, nfail, nvac);
@@ -393,6 +426,8 @@ hipError_t launch_batch_bc(Batch& b, int phase, int role) {
     return hipGetLastError();
 }
 hipError_t launch_batch_prim(Batch& b, int phase) {
+    EOS_DISPATCH_B(b, hipLaunchKernelGGL((kb_prim_check<EK, NCH>), dim3(b.nblocks[KB_CELLS]), dim3(256), 0,
+                                         b.stream, b.run, phase));
     EOS_DISPATCH_B(b, hipLaunchKernelGGL((kb_prim<EK, NCH>), dim3(b.nblocks[KB_CELLS]), dim3(256), 0, b.stream,
                                          b.run, phase));
     if (b.run.learned)

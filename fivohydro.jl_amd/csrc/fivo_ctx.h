@@ -56,7 +56,7 @@ struct DevScalars {
     FIVO_SCALAR(nvacuum);
     FIVO_SCALAR(npending);      // near-bound cells awaiting the inversion test
     FIVO_SCALAR(ncold);         // cells of the current primitive pass queued for the cooperative solve
-    FIVO_SCALAR(nresq);         // of a plain pass's queue, resumed solves (at the queue's far end)
+    FIVO_SCALAR(nresq);         // of a pass's queue, resumed solves (at the queue's far end)
     FIVO_SCALAR(nwarm);         // cells of the current primitive pass whose warm start iterates
     // cumulative
     FIVO_SCALAR(newton_iters);
@@ -207,7 +207,9 @@ struct EvView {
     double* U[4];
     DevScalars* sc;
     double* partials;     // 2 per interior block
-    PrimJob* jobs;        // queued cells of the event's primitive pass (count: sc->ncold)
+    PrimJob* jobs;        // queued cells of the event's primitive pass: sc->ncold seeded from the
+                          // front, sc->nresq resumed from the far end (jobs[Ntot - 1] down)
+    int* wlist;           // cells of the event's warm pass whose Newton iterates (count: sc->nwarm)
     int tiles_y, nmain;   // face tiling of the ensemble's tile shape
     int b0[KB_N];         // first block of the event in each grid kind
     int nb[KB_N];         // blocks of the event in each grid kind
