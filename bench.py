@@ -510,7 +510,9 @@ def kernel_alg_bytes(kernel, cfg, g, r, launches):
       k_relax      per interior cell: U read and the next buffer written (2 nv), the
                    primitives read (yT, alpha, mu, n, e, P, u^x, u^y: 8 and ok), the
                    six histories read and written (12)
-      k_bc         per ghost cell: nv doubles read and written"""
+      k_bc         per ghost cell: nv doubles read and written
+      k_stage_end  the ghost fill of k_bc, and on every stage but an attempt's last
+                   the four warm starts of every cell read and written (averaged)"""
     ncell = g.Nx * g.Ny
     nv = cfg["nv"]
     if kernel == "k_faces":
@@ -523,6 +525,9 @@ def kernel_alg_bytes(kernel, cfg, g, r, launches):
         return (8 * (2 * nv + 8 + 12) + 1) * ncell
     if kernel == "k_bc":
         return 8 * 2 * nv * (g.Ntot - ncell)
+    if kernel == "k_stage_end":
+        ns = 3 if cfg["integrator"] == "ssprk3" else 2
+        return 8 * 2 * nv * (g.Ntot - ncell) + 8 * 8 * g.Ntot * (ns - 1) / ns
     return 8 * nv * 2 * ncell
 
 
@@ -676,7 +681,7 @@ def kernel_times(lib, ctx):
     event pair around each launch on the library stream)."""
     tms = {}
     for cat, name in ((0, "k_prim"), (5, "k_prim_coop"), (6, "k_prim_seed"), (1, "k_faces"), (2, "k_bc"),
-                      (3, "k_relax"), (7, "k_background")):
+                      (3, "k_relax"), (7, "k_background"), (8, "k_stage_end")):
         tot = C.c_double(0)
         cnt = lib.kernel_time(ctx, cat, C.byref(tot))
         if cnt > 0:

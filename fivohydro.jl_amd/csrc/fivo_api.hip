@@ -215,19 +215,18 @@ int stage(Ctx& c, double* Ub, const double* Uold, double* Uout, double tau, doub
     return 0;
 }
 
-// One RK stage's launches up to its admissibility count, without reading it back:
-// the near-bound cells' inversion test always runs (a no-op when no cell is pending).
+// One RK stage's launches up to its face update, without reading anything back
+// (the first stage of an attempt also clears the per-pass scalars and fills the
+// ghosts of its input; k_stage_end does both for the stages after it).
 int stage_launch(Ctx& c, double* Ub, const double* Uold, double* Uout, double tau, double tau_state, double D,
-                 int combine, double wold, double wnew, int bc, bool bc_first) {
+                 int combine, double wold, double wnew, int bc, bool first) {
     int rc;
-    if ((rc = reset_scalars(c)) < 0) return rc;
-    if (bc_first && (rc = do_bc(c, Ub, bc)) < 0) return rc;
+    if (first && (rc = reset_scalars(c)) < 0) return rc;
+    if (first && (rc = do_bc(c, Ub, bc)) < 0) return rc;
     FaceLaunch f{};
     f.Ub = Ub; f.Uold = Uold; f.Uout = Uout; f.dUout = nullptr; f.mode = 1; f.combine = combine;
     f.tau = tau; f.tau_state = tau_state; f.D = D; f.wold = wold; f.wnew = wnew; f.force = nullptr;
     if ((rc = do_prim_faces(c, Ub, tau, f)) < 0) return rc;
-    if ((rc = do_bc(c, Uout, bc)) < 0) return rc;
-    HIPCHK(&c, launch_admissible_pending(c, Uout, tau_state));
     c.nrhs += 1;
     return FIVO_OK;
 }
@@ -240,9 +239,10 @@ void take_scalars(Ctx& c, const DevScalars& s) {
     c.last_vacuum = (int64_t)s.nvacuum;
 }
 
-// Every stage of one SSPRK attempt (no MOOD retries) launched back to back, the
-// device scalars copied to a pinned slot after each stage, one synchronisation for
-// the attempt instead of one per stage. *failed = the first stage whose output is
+// Every stage of one SSPRK attempt (no MOOD retries) launched back to back, each
+// ending in k_stage_end (ghost fill, pending test, the device scalars to a slot,
+// the warm starts to a rollback slot), the slots read back in one copy, one
+// synchronisation for the attempt instead of one per stage. *failed = the first stage whose output is
 // inadmissible (1-based), 0 if none. After a failure the later stages' changes that
 // the retry would see are rolled back -- the Newton warm starts (copied after each
 // stage but the last) and the device counters (from the failed stage's slot) -- so
@@ -257,16 +257,16 @@ int step_all_stages(Ctx& c, int integ, double tau, double D, int bc, int* failed
     const size_t wsn = 4 * (size_t)c.p.pitch;   // x0_yT, x0_phi, x0_ux, x0_uy: consecutive pitch blocks
     for (int k = 0; k < ns; ++k) {
         int rc;
+        double* out = k == 0 ? S1 : (k == 1 ? S2 : S3);
         if (k == 0) rc = stage_launch(c, U, U, S1, tau, tau + D, D, 0, 1.0, 0.0, bc, true);
         else if (integ == FIVO_SSPRK3 && k == 1) rc = stage_launch(c, S1, U, S2, tau + D, tau + D, D, 1, 0.75, 0.25, bc, false);
         else if (integ == FIVO_SSPRK3) rc = stage_launch(c, S2, U, S3, tau + 0.5 * D, tau + D, D, 2, 0.0, 0.0, bc, false);
         else rc = stage_launch(c, S1, U, S2, tau + D, tau + D, D, 1, 0.5, 0.5, bc, false);
         if (rc < 0) return rc;
-        HIPCHK(&c, hipMemcpyAsync(&c.sc_stage_host[k], c.sc, sizeof(DevScalars), hipMemcpyDeviceToHost, c.stream));
-        if (k + 1 < ns)
-            HIPCHK(&c, hipMemcpyAsync(c.ws + k * wsn, c.w.x0_yT, sizeof(double) * wsn, hipMemcpyDeviceToDevice,
-                                      c.stream));
+        Timer t(c, TK_TAIL);
+        HIPCHK(&c, launch_stage_end(c, out, tau + D, bc, k, k + 1 < ns ? c.ws + k * wsn : nullptr, true));
     }
+    HIPCHK(&c, hipMemcpyAsync(c.sc_stage_host, c.sc_slots, ns * sizeof(DevScalars), hipMemcpyDeviceToHost, c.stream));
     HIPCHK(&c, hipStreamSynchronize(c.stream));
     resolve_timing(c);
     *failed = 0;
@@ -280,6 +280,7 @@ int step_all_stages(Ctx& c, int integ, double tau, double D, int bc, int* failed
         c.nrhs = nrhs0 + *failed;
     }
     take_scalars(c, c.sc_stage_host[last]);
+    c.pass_clean = *failed == 0;
     return FIVO_OK;
 }
 
@@ -293,6 +294,7 @@ int step_dev(Ctx& c, int integ, double tau, double dtau, int bc, int max_halve, 
     double D = dtau;
     int rc;
     const bool batch = integ == FIVO_SSPRK3 || max_mood == 0;
+    c.pass_clean = false;
     for (int h = 0; h <= max_halve; ++h) {
         if (batch && h < max_halve) {
             int failed = 0;
@@ -615,9 +617,12 @@ int fivo_create(const fivo_grid_desc* g, const fivo_model_desc* m, fivo_ctx** ou
     c->jobs = reinterpret_cast<PrimJob*>(b); b += sizeof(PrimJob) * P;
     c->wlist = reinterpret_cast<int*>(b); b += sizeof(int) * P;
     c->dirty = b; b += c->ntiles_max;
+    b += (4 - (reinterpret_cast<uintptr_t>(b) & 3)) & 3;   // within the pool's 256 spare bytes
+    c->tail_done = reinterpret_cast<unsigned int*>(b); b += sizeof(unsigned int);
     if ((e = hipMalloc(&c->sc, sizeof(DevScalars))) != hipSuccess ||
         (e = hipHostMalloc(&c->sc_host, sizeof(DevScalars), hipHostMallocDefault)) != hipSuccess ||
         (e = hipHostMalloc(&c->sc_stage_host, 4 * sizeof(DevScalars), hipHostMallocDefault)) != hipSuccess ||
+        (e = hipMalloc(&c->sc_slots, 4 * sizeof(DevScalars))) != hipSuccess ||
         (e = hipHostMalloc(&c->partials_host, sizeof(double) * 2 * (size_t)c->npartials, hipHostMallocDefault)) !=
             hipSuccess ||
         (e = hipEventCreate(&c->ev[0])) != hipSuccess || (e = hipEventCreate(&c->ev[1])) != hipSuccess ||
@@ -648,6 +653,7 @@ void fivo_destroy(fivo_ctx* ctx) {
     if (c->sc) (void)hipFree(c->sc);
     if (c->sc_host) (void)hipHostFree(c->sc_host);
     if (c->sc_stage_host) (void)hipHostFree(c->sc_stage_host);
+    if (c->sc_slots) (void)hipFree(c->sc_slots);
     if (c->partials_host) (void)hipHostFree(c->partials_host);
     for (auto& ev : c->ev) if (ev) (void)hipEventDestroy(ev);
     for (auto& ev : c->evpool) (void)hipEventDestroy(ev);
@@ -982,7 +988,8 @@ int fivo_run(fivo_ctx* ctx, const fivo_run_opts* o, fivo_run_result* r) {
         nsteps += 1;
         if (dissip) {
             // update_primitives_2d! + relax_dissipative_2d!; amax stays the last RHS's
-            if ((rc = reset_scalars(c)) < 0) return rc;
+            if (!c.pass_clean && (rc = reset_scalars(c)) < 0) return rc;
+            c.pass_clean = false;
             if ((rc = do_bc(c, c.U[c.cur], o->bc)) < 0) return rc;
             if ((rc = do_prim(c, c.U[c.cur], tau)) < 0) return rc;
             if ((rc = relax_dev(c, tau, used)) < 0) return rc;

@@ -95,8 +95,9 @@ struct DevWork {
 };
 
 // timing categories: warm primitive kernel, fused faces, ghost fill, relaxation,
-// everything, cooperative primitive kernel, learned seeds
-enum { TK_PRIM = 0, TK_FACES = 1, TK_BC = 2, TK_RELAX = 3, TK_ALL = 4, TK_COOP = 5, TK_SEED = 6, TK_BG = 7, TK_N = 8 };
+// everything, cooperative primitive kernel, learned seeds, background slice, stage end
+enum { TK_PRIM = 0, TK_FACES = 1, TK_BC = 2, TK_RELAX = 3, TK_ALL = 4, TK_COOP = 5, TK_SEED = 6, TK_BG = 7,
+       TK_TAIL = 8, TK_N = 9 };
 
 struct Ctx {
     fivo_grid_desc gd;
@@ -126,6 +127,8 @@ struct Ctx {
     // warm starts after stages 1 and 2 (2 x 4 pitch doubles, the rollback of a failed
     // stage); the shear projection's partials, pinned, read at the next synchronisation
     DevScalars* sc_stage_host = nullptr;
+    DevScalars* sc_slots = nullptr;          // device copies of the stage scalars (k_stage_end), 4 slots
+    unsigned int* tail_done = nullptr;       // k_stage_end's finished-block count (zero between launches)
     double* ws = nullptr;
     double* partials_host = nullptr;
     double amax_host = 0.0;         // last RHS amax, as read back
@@ -146,6 +149,9 @@ struct Ctx {
     // run accounting
     int64_t nrhs = 0, nhalvings = 0, nmood = 0;
     int64_t last_primfail = 0, last_vacuum = 0;
+    // the per-pass device scalars are zero (an accepted all-stage attempt cleared
+    // them after its last stage), so the refresh pass that follows needs no clear
+    bool pass_clean = false;
 };
 
 // ---- launchers (fivo_kernels.hip) ------------------------------------------
@@ -171,6 +177,9 @@ hipError_t launch_expand(Ctx& c, const uint8_t* in, uint8_t* out);
 hipError_t launch_sanitize(Ctx& c, double* U, double tau, int use_P);
 hipError_t launch_admissible(Ctx& c, const double* U, double tau, int use_work, uint8_t* bad_out);
 hipError_t launch_admissible_pending(Ctx& c, const double* U, double tau);
+// ghost fill + pending test of a stage output, its scalars to sc_slots[slot], the
+// warm starts to ws_dst (nullable), per-pass scalars cleared (clear): one launch
+hipError_t launch_stage_end(Ctx& c, double* U, double tau, int bc, int slot, double* ws_dst, bool clear);
 hipError_t launch_init_prims(Ctx& c, double* U, double tau0, const double* const* prims, unsigned long long* nbad);
 hipError_t launch_fill(Ctx& c, double* p, double v, int64_t n);
 hipError_t launch_hotspot_sum(Ctx& c, double* s_out, int nspots, const double* spots, double sigma, double* part);

@@ -106,6 +106,63 @@ __global__ __launch_bounds__(256) void k_admissible_pending(const double* U, Par
     pending_cell<EK, NCH>(U, p, tau, bad, sc, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
+// The end of one RK stage of the plain loop in one launch: blocks [0, nbc) fill
+// the ghost cells of the stage output (bc_cell), blocks [nbc, nbc + npd) settle
+// its pending near-bound cells (pending_cell, striding over the interior), the
+// others copy the Newton warm starts (wsn doubles from x0_yT on) to the stage's
+// rollback slot when ws_dst is set. Once the scalars are final -- at once when no
+// cell is pending (the first pending block), else after the last pending block --
+// they are copied to the stage's slot and, with clear, the per-pass ones
+// (amax_bits .. nwarm) are zeroed for the next stage.
+template <int EK, bool NCH>
+__global__ __launch_bounds__(256) void k_stage_end(double* __restrict__ U, Params p, int periodic, double tau,
+                                                   uint8_t* bad, DevScalars* sc, DevScalars* slot,
+                                                   const double* __restrict__ ws_src, double* __restrict__ ws_dst,
+                                                   int64_t wsn, int nbc, int npd, int clear, unsigned int* done) {
+    const int b = blockIdx.x;
+    if (b < nbc) {
+        bc_cell(U, p, periodic, b * 256 + threadIdx.x);
+        return;
+    }
+    if (b >= nbc + npd) {
+        if (!ws_dst) return;
+        const int64_t n2 = wsn / 2;
+        const double2* __restrict__ s2 = reinterpret_cast<const double2*>(ws_src);
+        double2* __restrict__ d2 = reinterpret_cast<double2*>(ws_dst);
+        const int64_t stride = (int64_t)(gridDim.x - nbc - npd) * 256;
+        for (int64_t k = (int64_t)(b - nbc - npd) * 256 + threadIdx.x; k < n2; k += stride) d2[k] = s2[k];
+        return;
+    }
+    // pending blocks: every block of the kernel reads npending before any writes it
+    const bool pend = sc->npending != 0;
+    __shared__ bool last;
+    if (pend) {
+        const int n = p.Nx * p.Ny;
+        for (int t = (b - nbc) * 256 + threadIdx.x; t < n; t += npd * 256) pending_cell<EK, NCH>(U, p, tau, bad, sc, t);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            __threadfence();
+            last = atomicAdd(done, 1u) == (unsigned int)npd - 1;
+        }
+        __syncthreads();
+        if (!last) return;
+        __threadfence();
+        if (threadIdx.x == 0) *done = 0;
+    } else if (b != nbc) {
+        return;
+    }
+    constexpr int NW = (int)(sizeof(DevScalars) / sizeof(unsigned long long));
+    const unsigned long long* src = reinterpret_cast<const unsigned long long*>(sc);
+    unsigned long long* dst = reinterpret_cast<unsigned long long*>(slot);
+    for (int k = threadIdx.x; k < NW; k += 256) dst[k] = __hip_atomic_load(src + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    if (clear) {
+        unsigned long long* f0 = &sc->amax_bits;
+        const int nz = (int)(&sc->nwarm - f0) + 1;
+        for (int k = threadIdx.x; k < nz; k += 256) f0[k] = 0;
+    }
+}
+
 __global__ void k_history(DevWork w, int n) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) history_cell(w, t);
@@ -414,6 +471,18 @@ hipError_t launch_admissible(Ctx& c, const double* U, double tau, int use_work, 
 hipError_t launch_admissible_pending(Ctx& c, const double* U, double tau) {
     EOS_DISPATCH(c, hipLaunchKernelGGL((k_admissible_pending<EK, NCH>), dim3(nblk(c.p.Nx * c.p.Ny, 256)), dim3(256),
                                        0, c.stream, U, c.p, tau, c.w.bad, c.sc));
+    return hipGetLastError();
+}
+
+hipError_t launch_stage_end(Ctx& c, double* U, double tau, int bc, int slot, double* ws_dst, bool clear) {
+    const Params& p = c.p;
+    const int nbc = nblk(2 * (int64_t)p.ng * p.Nytot + 2 * (int64_t)p.ng * p.Nx, 256);
+    const int npd = std::min(nblk((int64_t)p.Nx * p.Ny, 256), 64);
+    const int ncp = ws_dst ? 256 : 0;
+    const int64_t wsn = 4 * (int64_t)p.pitch;   // x0_yT, x0_phi, x0_ux, x0_uy: consecutive pitch blocks
+    EOS_DISPATCH(c, hipLaunchKernelGGL((k_stage_end<EK, NCH>), dim3(nbc + npd + ncp), dim3(256), 0, c.stream, U, p,
+                                       bc == FIVO_BC_PERIODIC ? 1 : 0, tau, c.w.bad, c.sc, c.sc_slots + slot,
+                                       c.w.x0_yT, ws_dst, wsn, nbc, npd, clear ? 1 : 0, c.tail_done));
     return hipGetLastError();
 }
 

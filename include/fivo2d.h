@@ -247,8 +247,9 @@ int fivo_background_slice(fivo_ctx* ctx, int32_t k, int32_t nt, double* T, doubl
 /* Total device time (ms) of the kernels by category since fivo_enable_timing,
  * measured with HIP events on the context stream: 0 = primitive pass (warm
  * kernel), 1 = faces+update, 2 = boundary fill, 3 = relaxation, 4 = everything,
- * 5 = primitive pass (cooperative kernel), 6 = learned seeds. Returns the count
- * of launches. */
+ * 5 = primitive pass (cooperative kernel), 6 = learned seeds, 7 = background
+ * slice, 8 = stage end (ghost fill, pending test and scalar copies of an RK stage
+ * in one launch). Returns the count of launches. */
 int64_t fivo_kernel_time(const fivo_ctx* ctx, int category, double* total_ms);
 int fivo_enable_timing(fivo_ctx* ctx, int on);
 /* Histograms of the primitive passes of the last fivo_run (cumulative over the run),
