@@ -988,9 +988,11 @@ int fivo_run(fivo_ctx* ctx, const fivo_run_opts* o, fivo_run_result* r) {
         nsteps += 1;
         if (dissip) {
             // update_primitives_2d! + relax_dissipative_2d!; amax stays the last RHS's
+            // after an accepted all-stage attempt the per-pass scalars are clear and
+            // the last stage-end kernel has filled the new state's ghosts
             if (!c.pass_clean && (rc = reset_scalars(c)) < 0) return rc;
+            if (!c.pass_clean && (rc = do_bc(c, c.U[c.cur], o->bc)) < 0) return rc;
             c.pass_clean = false;
-            if ((rc = do_bc(c, c.U[c.cur], o->bc)) < 0) return rc;
             if ((rc = do_prim(c, c.U[c.cur], tau)) < 0) return rc;
             if ((rc = relax_dev(c, tau, used)) < 0) return rc;
             HIPCHK(&c, hipMemcpyAsync(&c.sc_stage_host[3], c.sc, sizeof(DevScalars), hipMemcpyDeviceToHost,

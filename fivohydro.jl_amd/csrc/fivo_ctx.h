@@ -149,8 +149,8 @@ struct Ctx {
     // run accounting
     int64_t nrhs = 0, nhalvings = 0, nmood = 0;
     int64_t last_primfail = 0, last_vacuum = 0;
-    // the per-pass device scalars are zero (an accepted all-stage attempt cleared
-    // them after its last stage), so the refresh pass that follows needs no clear
+    // an accepted all-stage attempt left the per-pass device scalars zero and the
+    // new state's ghosts filled, so the refresh pass that follows needs neither
     bool pass_clean = false;
 };
 
