@@ -63,12 +63,13 @@ TRAFFIC = {
     ("c3", "k_faces"): (2.0 * 42366.4 + 34257.4) * 1024,
     ("c3", "k_relax"): (2.0 * 44982.5 + 67038.4) * 1024,
     ("c3", "k_bc"): (2.0 * 1254.5 + 1159.0) * 1024,
+    ("c3", "k_stage_end"): (2.0 * 4070.4 + 6755.7) * 1024,
 }
 
 TRAFFIC_SOURCE = ("profile constant: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this workload, "
                   "scaled by a 512 MiB device copy (profiles/full_sectors_512_kernel_stats.md)")
 FP64_GFLOP = {("c3", "k_prim_coop"): 0.718, ("c3", "k_prim"): 0.150 + 0.609, ("c3", "k_faces"): 1.347,
-              ("c3", "k_relax"): 0.554, ("c3", "k_bc"): 0.0}
+              ("c3", "k_relax"): 0.554, ("c3", "k_bc"): 0.0, ("c3", "k_stage_end"): 0.0}
 FP64_PEAK_TFLOPS = 78.6
 
 # Workloads: (description, builder). Algorithmic HBM bytes per cell-update and per
