@@ -15,6 +15,14 @@ ORACLE_LIB = os.path.join(REPO, "oracle", "liboracle2d.so")
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU")
+    # Under pytest-xdist every worker would start one OpenMP thread per core
+    # for the oracle (and torch its own pool): share the cores out instead, or
+    # 8 workers on 8 cores run 60+ busy threads and the suite takes 10x longer.
+    workers = os.environ.get("PYTEST_XDIST_WORKER_COUNT")
+    if workers and int(workers) > 1:
+        per = max(1, (os.cpu_count() or 1) // int(workers))
+        os.environ["OMP_NUM_THREADS"] = str(per)
+        os.environ["MKL_NUM_THREADS"] = str(per)
 
 
 def gpu_available():
